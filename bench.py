@@ -1,0 +1,214 @@
+"""Throughput of the FP64 MPM step at 16M particles on one MI355X.
+
+Workload (the 16M-particle throughput configuration): a 200x100x100-cell
+block, 8 points per cell (16,000,000), on the 512x256x256 grid with
+h = 1/64 m, dt = 1e-5 s, E = 9e9 Pa, nu = 0.3, friction angle 60 deg,
+cohesion 2e6 Pa, sphere indenter r = 0.1 m at 1 m/s. One step = one substep
+of the reference's Model3D::Step loop (reset, P2G, grid update, G2P with the
+return mapping), with the brick re-sort every --sort-period steps inside the
+timed region. Points are uploaded before timing (HBM-resident).
+
+Prints one JSON line: particle-updates/s, the HBM roofline of the dominant
+kernel (algorithmic bytes / HIP-event duration on the library's stream) and
+the host-core baseline (the oracle on this host's cores, a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes (FP64 fields of the reference SoA):
+# P2G reads utility, pos, vel, Fe, Bp = 25 doubles; G2P reads 14 and writes 26;
+# per node holding mass: flush 32 B (P2G), update 32 B read + 24 B write,
+# G2P 24 B read, reset 32 B.
+B_P2G_PT, B_G2P_PT = 200, 320
+B_P2G_NODE, B_UPD_NODE, B_G2P_NODE, B_RESET_NODE = 32, 56, 24, 32
+B_SORT_PT = 24 + 8 + 4 * 16 + 416
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, prms, hs, max_seconds: float, max_steps: int, threads: int) -> dict:
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as orc
+    from icempm_3dm_amd import capi
+
+    shape, center, vel = scene.indenter()
+    ind = capi.Indenter.make(shape, center, vel)
+    soa = hs.copy()
+    grid = orc.grid_zeros(prms)
+    bins = np.zeros(prms.IndenterArraySize())
+    t_sim = 0.0
+    # one untimed step (page-in), then timed steps until the budget is spent
+    _, t_sim = orc.step(prms, ind, soa.host_buffer, soa.capacity, soa.size, grid, bins, 1, t_sim, 0, threads)
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        _, t_sim = orc.step(prms, ind, soa.host_buffer, soa.capacity, soa.size, grid, bins, 1, t_sim, 1 + steps,
+                            threads)
+        steps += 1
+        if time.perf_counter() - t0 >= max_seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": soa.size * steps / dt,
+        "unit": "particle-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} steps of the same {soa.size}-point workload after 1 warm-up step "
+                  f"(oracle/mpm_oracle.cpp, OpenMP, {threads} threads, {os.uname().machine}); "
+                  f"{dt:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--scene", default="block_16m")
+    ap.add_argument("--sort-period", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs the "
+              "single-partition step on one device)", file=sys.stderr)
+        return 2
+
+    from icempm_3dm_amd import GPUImplementation6, capi, scenes
+
+    if capi.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+    scene = scenes.SCENES[args.scene]
+    prms = scenes.make_params(scene)
+    log(f"bench: generating {scene.name} ({scene.n_points} points)")
+    hs = scenes.make_points(scene)
+    n = hs.size
+
+    g = GPUImplementation6(0)
+    g.hssoa = hs
+    g.allocate_arrays(prms, hs.capacity)
+    g.update_constants(prms)
+    shape, center, vel = scene.indenter()
+    g.set_indenter(shape, center, vel)
+    g.set_sort_period(args.sort_period)
+    log("bench: uploading")
+    g.transfer_ponts_to_device()
+
+    t_sim, step = 0.0, 0
+    log(f"bench: {args.warmup} warm-up steps")
+    t_sim = g.step(args.warmup, t_sim, step)
+    step += args.warmup
+    g.synchronize()
+    if g.device_error_code() != 0:
+        print(f"bench: device error {g.device_error_code()} during warm-up", file=sys.stderr)
+        return 1
+
+    g.enable_timing(True)
+    g.reset_timings()
+    log(f"bench: {args.steps} timed steps")
+    g.synchronize()
+    t0 = time.perf_counter()
+    t_sim = g.step(args.steps, t_sim, step)
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    step += args.steps
+    timings = g.read_timings()
+    g.enable_timing(False)
+    err = g.device_error_code()
+
+    # nodes holding mass after the last step (for the per-node algorithmic bytes)
+    g.reset_grid()
+    g.p2g()
+    grid = g.download_grid()
+    active_nodes = int(np.count_nonzero(grid[0]))
+    del grid
+    g.close()
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n * args.steps / elapsed
+    per_launch_bytes = {
+        "p2g": B_P2G_PT * n + B_P2G_NODE * active_nodes,
+        "update": B_UPD_NODE * active_nodes,
+        "g2p": B_G2P_PT * n + B_G2P_NODE * active_nodes,
+        "reset": B_RESET_NODE * active_nodes,
+        "sort": B_SORT_PT * n,
+    }
+    phases = {}
+    for name, (ms, cnt) in timings.items():
+        if cnt == 0:
+            continue
+        avg_ms = ms / cnt
+        phases[name] = {"launches": cnt, "avg_ms": avg_ms, "alg_bytes": per_launch_bytes[name],
+                        "alg_GBs": per_launch_bytes[name] / (avg_ms * 1e-3) / 1e9}
+    dominant = max((k for k in phases if k != "sort"), key=lambda k: phases[k]["avg_ms"] * phases[k]["launches"])
+    d = phases[dominant]
+    step_bytes = (B_P2G_PT + B_G2P_PT) * n + (B_P2G_NODE + B_UPD_NODE + B_G2P_NODE + B_RESET_NODE) * active_nodes
+    roofline = {
+        "bound": "hbm",
+        "kernel": dominant,
+        "achieved": round(d["alg_GBs"], 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(d["alg_GBs"] / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "step_alg_GBs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"bench: host-core baseline ({args.cpu_threads} threads, <= {args.cpu_seconds} s)")
+        cpu = cpu_baseline(scene, prms, hs, args.cpu_seconds, args.cpu_max_steps, args.cpu_threads)
+
+    out = {
+        "metric": "particle-updates/s and % of HBM roofline at 16M particles",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{scene.name}: {n} points ({scene.ppc_axis ** 3}/cell) on a "
+                        f"{scene.grid[0]}x{scene.grid[1]}x{scene.grid[2]} grid, h=1/64, dt={scene.dt}, "
+                        f"sphere indenter, brick re-sort every {args.sort_period} steps",
+            "particles": n,
+            "grid": list(scene.grid),
+            "active_nodes": active_nodes,
+            "device_error": err,
+        },
+        "roofline": roofline,
+        "phases": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                   for k, v in phases.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if err == 0 else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

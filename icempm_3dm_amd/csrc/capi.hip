@@ -1,0 +1,709 @@
+// capi.hip — the C-ABI of include/icempm_3dm.h: one context per
+// (device, partition), owning its stream, device particle storage, grid,
+// indenter accumulators and sort scratch. Host code only; kernels live in
+// kernels_*.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "icempm_3dm.h"
+#include "kernels.h"
+
+using namespace icempm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define ICEMPM_HIP(expr)                                                                                      \
+    do {                                                                                                     \
+        hipError_t _e = (expr);                                                                              \
+        if (_e != hipSuccess)                                                                                \
+            return fail(ICEMPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+    } while (0)
+
+#define ICEMPM_TRY(expr)                                                                                      \
+    do {                                                                                                     \
+        int _r = (expr);                                                                                     \
+        if (_r != ICEMPM_OK) return _r;                                                                      \
+    } while (0)
+
+struct TimedLaunch {
+    int phase;
+    hipEvent_t start, stop;
+};
+
+} // namespace
+
+struct icempm_ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+
+    icempm_SimParams3D prms{};
+    bool have_params = false;
+    icempm_Indenter ind{};
+    bool ind_custom = false;
+    KParams kp{};
+
+    // particles
+    bool allocated = false;
+    int capacity = 0;
+    long long pitch = 0;
+    int n = 0;
+    double *pts[2] = {nullptr, nullptr};
+    int cur_pts = 0;
+    int32_t *slot[2] = {nullptr, nullptr};
+    uint32_t *keys[2] = {nullptr, nullptr};
+    int32_t *vals[2] = {nullptr, nullptr};
+    int32_t *scratch = nullptr;
+    int32_t *seg_start = nullptr;
+    int32_t *seg_brick = nullptr;
+    int32_t *nseg = nullptr;
+    int32_t *work = nullptr; // crushed-particle worklist (+1 count)
+    int p2g_blocks = 2048;
+
+    // grid
+    double *grid = nullptr;
+    long long nbricks = 0;
+    int32_t *brick_flag = nullptr;
+    int32_t *brick_list[2] = {nullptr, nullptr};
+    int32_t *brick_count = nullptr;
+    int cur_list = 0;
+    double *bins = nullptr;
+    long long nbins = 0;
+    int *err = nullptr;
+    double *grid_xmajor = nullptr; // lazily allocated for download_grid
+
+    int sort_period = 0;
+    long long substeps_since_sort = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    double ms[ICEMPM_N_PHASES] = {0};
+    long long launches[ICEMPM_N_PHASES] = {0};
+
+    PtsView view(int which) const { return PtsView{pts[which], pitch}; }
+    GridState gstate() const {
+        GridState g;
+        g.grid = grid;
+        g.brick_flag = brick_flag;
+        g.brick_list[0] = brick_list[0];
+        g.brick_list[1] = brick_list[1];
+        g.brick_count = brick_count;
+        return g;
+    }
+};
+
+namespace {
+
+int set_device(icempm_ctx *ctx) {
+    ICEMPM_HIP(hipSetDevice(ctx->device));
+    return ICEMPM_OK;
+}
+
+hipEvent_t take_event(icempm_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int resolve_timings(icempm_ctx *ctx) {
+    if (ctx->pending.empty()) return ICEMPM_OK;
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &tl : ctx->pending) {
+        float msf = 0.f;
+        ICEMPM_HIP(hipEventElapsedTime(&msf, tl.start, tl.stop));
+        ctx->ms[tl.phase] += msf;
+        ctx->launches[tl.phase] += 1;
+        ctx->event_pool.push_back(tl.start);
+        ctx->event_pool.push_back(tl.stop);
+    }
+    ctx->pending.clear();
+    return ICEMPM_OK;
+}
+
+// Brackets one phase with events when timing is on.
+struct PhaseTimer {
+    icempm_ctx *ctx;
+    int phase;
+    hipEvent_t start = nullptr;
+    PhaseTimer(icempm_ctx *c, int ph) : ctx(c), phase(ph) {
+        if (ctx->timing) {
+            start = take_event(ctx);
+            if (start && hipEventRecord(start, ctx->stream) != hipSuccess) start = nullptr;
+        }
+    }
+    int finish() {
+        if (!ctx->timing || !start) return ICEMPM_OK;
+        hipEvent_t stop = take_event(ctx);
+        if (!stop) return fail(ICEMPM_ERR_HIP, "hipEventCreate failed");
+        ICEMPM_HIP(hipEventRecord(stop, ctx->stream));
+        ctx->pending.push_back({phase, start, stop});
+        if (ctx->pending.size() > 8192) return resolve_timings(ctx);
+        return ICEMPM_OK;
+    }
+};
+
+void fill_kparams(icempm_ctx *ctx) {
+    const icempm_SimParams3D &P = ctx->prms;
+    KParams &k = ctx->kp;
+    k.gx = P.GridXTotal;
+    k.gy = P.GridY;
+    k.gz = P.GridZ;
+    k.nbx = (P.GridXTotal + 3) / 4;
+    k.nby = (P.GridY + 3) / 4;
+    k.nbz = (P.GridZ + 3) / 4;
+    k.grid_pitch = (long long)k.nbx * k.nby * k.nbz * kBrickNodes;
+    k.h = P.cellsize;
+    k.h_inv = P.cellsize_inv;
+    k.dt = P.InitialTimeStep;
+    k.mass_solid = P.ParticleMass;
+    k.mass_liquid = P.LiquidParticleMass;
+    k.dt_vol_Dpinv = P.dt_vol_Dpinv;
+    k.dt_gravity = P.dt_Gravity;
+    k.vmax = P.vmax;
+    k.vmax_sq = P.vmax_squared;
+    k.mu = P.mu;
+    k.kappa = P.kappa;
+    k.ice_comp = P.IceCompressiveStrength;
+    k.ice_tens = P.IceTensileStrength;
+    k.ice_shear = P.IceShearStrength;
+    k.ice_tens2 = P.IceTensileStrength2;
+    k.nacc_beta = P.NACC_beta;
+    k.tan_phi = P.DP_tan_phi;
+    k.dp_threshold_p = P.DP_threshold_p;
+    k.grain_var = P.GrainVariability;
+    k.drag = P.DragGlobal;
+    k.setup_type = P.SetupType;
+    k.ind_subdiv = P.IndenterSubdivisions;
+    k.ind_rsq = P.IndRSq;
+    if (!ctx->ind_custom) {
+        // the reference law (model_3d.cpp:38, mpm_kernels.cuh:154)
+        ctx->ind.shape = ICEMPM_INDENTER_CYLINDER_Z;
+        ctx->ind.center_initial[0] = P.indenter_x_initial;
+        ctx->ind.center_initial[1] = P.indenter_y_initial;
+        ctx->ind.center_initial[2] = 0.0;
+        ctx->ind.velocity[0] = P.IndVelocity;
+        ctx->ind.velocity[1] = 0.0;
+        ctx->ind.velocity[2] = 0.0;
+    }
+    k.ind_shape = ctx->ind.shape;
+    for (int d = 0; d < 3; d++) k.ind_vel[d] = ctx->ind.velocity[d];
+}
+
+template <typename T> int dev_alloc(T **p, size_t count, const char *what) {
+    hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(ICEMPM_ERR_HIP, std::string("hipMalloc ") + what + " (" + std::to_string(count * sizeof(T)) +
+                                        " bytes): " + hipGetErrorString(e));
+    }
+    return ICEMPM_OK;
+}
+
+void free_all(icempm_ctx *ctx) {
+    auto f = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    for (int i = 0; i < 2; i++) {
+        f(ctx->pts[i]);
+        f(ctx->slot[i]);
+        f(ctx->keys[i]);
+        f(ctx->vals[i]);
+        f(ctx->brick_list[i]);
+        ctx->pts[i] = nullptr;
+        ctx->slot[i] = nullptr;
+        ctx->keys[i] = nullptr;
+        ctx->vals[i] = nullptr;
+        ctx->brick_list[i] = nullptr;
+    }
+    f(ctx->scratch);
+    f(ctx->seg_start);
+    f(ctx->seg_brick);
+    f(ctx->nseg);
+    f(ctx->work);
+    f(ctx->grid);
+    f(ctx->brick_flag);
+    f(ctx->brick_count);
+    f(ctx->bins);
+    f(ctx->err);
+    f(ctx->grid_xmajor);
+    ctx->scratch = ctx->seg_start = ctx->seg_brick = ctx->nseg = ctx->work = nullptr;
+    ctx->grid = nullptr;
+    ctx->brick_flag = ctx->brick_count = nullptr;
+    ctx->bins = nullptr;
+    ctx->err = nullptr;
+    ctx->grid_xmajor = nullptr;
+    ctx->allocated = false;
+}
+
+int check_ready(icempm_ctx *ctx) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    if (!ctx->allocated) return fail(ICEMPM_ERR_NOT_READY, "icempm_allocate has not been called");
+    if (!ctx->have_params) return fail(ICEMPM_ERR_NOT_READY, "icempm_update_constants has not been called");
+    return set_device(ctx);
+}
+
+// Regroups device points by brick key and rebuilds the brick segments.
+int brick_sort(icempm_ctx *ctx) {
+    PhaseTimer tm(ctx, ICEMPM_PHASE_SORT);
+    const int n = ctx->n;
+    if (n > 0) {
+        ICEMPM_HIP(launch_cell_keys(ctx->kp, ctx->view(ctx->cur_pts), n, 1, ctx->keys[0], ctx->vals[0], ctx->stream));
+        ICEMPM_HIP(radix_sort_pairs(ctx->keys[0], ctx->vals[0], ctx->keys[1], ctx->vals[1], n, ctx->scratch,
+                                    ctx->stream));
+        const int nxt = ctx->cur_pts ^ 1;
+        ICEMPM_HIP(launch_permute(ctx->view(ctx->cur_pts), ctx->view(nxt), ctx->slot[ctx->cur_pts], ctx->slot[nxt],
+                                  ctx->vals[0], n, ctx->stream));
+        ctx->cur_pts = nxt;
+    }
+    ICEMPM_HIP(launch_brick_segments(ctx->keys[0], n, ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->scratch,
+                                     ctx->stream));
+    ctx->substeps_since_sort = 0;
+    return tm.finish();
+}
+
+int do_reset_grid(icempm_ctx *ctx) {
+    PhaseTimer tm(ctx, ICEMPM_PHASE_RESET);
+    ICEMPM_HIP(launch_reset_grid(ctx->kp, ctx->gstate(), ctx->cur_list, ctx->stream));
+    ctx->cur_list ^= 1;
+    return tm.finish();
+}
+
+int do_p2g(icempm_ctx *ctx) {
+    PhaseTimer tm(ctx, ICEMPM_PHASE_P2G);
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    ICEMPM_HIP(launch_p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->gstate(), ctx->cur_list, ctx->err,
+                          ctx->p2g_blocks, ctx->stream));
+    return tm.finish();
+}
+
+int do_update(icempm_ctx *ctx, double cx, double cy, double cz) {
+    PhaseTimer tm(ctx, ICEMPM_PHASE_UPDATE);
+    ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(), ctx->cur_list, cx, cy, cz, ctx->bins, ctx->stream));
+    return tm.finish();
+}
+
+int do_g2p(icempm_ctx *ctx, int record_pq) {
+    PhaseTimer tm(ctx, ICEMPM_PHASE_G2P);
+    ICEMPM_HIP(launch_g2p(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, ctx->grid, record_pq, ctx->err, ctx->work + 1,
+                          ctx->work, ctx->stream));
+    ctx->substeps_since_sort++;
+    return tm.finish();
+}
+
+__global__ void k_fill_double(double *p, long long n, double v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_int(int *p, long long n, int v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int zero_bins(icempm_ctx *ctx) {
+    if (ctx->nbins > 0) {
+        k_fill_double<<<(unsigned)((ctx->nbins + 255) / 256), 256, 0, ctx->stream>>>(ctx->bins, ctx->nbins, 0.0);
+        ICEMPM_HIP(hipGetLastError());
+    }
+    return ICEMPM_OK;
+}
+
+int clear_err(icempm_ctx *ctx) {
+    k_fill_int<<<1, 1, 0, ctx->stream>>>(ctx->err, 1, 0);
+    ICEMPM_HIP(hipGetLastError());
+    return ICEMPM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *icempm_last_error(void) { return g_last_error.c_str(); }
+
+const char *icempm_version(void) { return "icempm_3dm_amd 0.1.0 (gfx950)"; }
+
+int icempm_get_device_count(int *count) {
+    if (!count) return fail(ICEMPM_ERR_INVALID_ARG, "null count");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return ICEMPM_OK;
+}
+
+int icempm_create(int device, icempm_ctx **out) {
+    if (!out) return fail(ICEMPM_ERR_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(ICEMPM_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(ICEMPM_ERR_INVALID_ARG, "device index out of range");
+    ICEMPM_HIP(hipSetDevice(device));
+    icempm_ctx *ctx = new icempm_ctx();
+    ctx->device = device;
+    hipError_t e = hipStreamCreate(&ctx->stream);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(ICEMPM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->p2g_blocks = prop.multiProcessorCount * 8;
+    *out = ctx;
+    return ICEMPM_OK;
+}
+
+int icempm_destroy(icempm_ctx *ctx) {
+    if (!ctx) return ICEMPM_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &tl : ctx->pending) {
+        (void)hipEventDestroy(tl.start);
+        (void)hipEventDestroy(tl.stop);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    free_all(ctx);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ICEMPM_OK;
+}
+
+int icempm_allocate(icempm_ctx *ctx, const icempm_SimParams3D *prms, int n_points_capacity) {
+    if (!ctx || !prms) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (n_points_capacity < 0) return fail(ICEMPM_ERR_INVALID_ARG, "negative capacity");
+    if (prms->GridXTotal < 3 || prms->GridY < 3 || prms->GridZ < 3)
+        return fail(ICEMPM_ERR_INVALID_ARG, "grid must be at least 3 nodes per axis");
+    if (prms->IndenterSubdivisions < 0) return fail(ICEMPM_ERR_INVALID_ARG, "negative IndenterSubdivisions");
+    if (prms->GridXTotal > 1024 || prms->GridY > 1024 || prms->GridZ > 1024)
+        return fail(ICEMPM_ERR_INVALID_ARG, "grid is limited to 1024 nodes per axis");
+    ICEMPM_TRY(set_device(ctx));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    free_all(ctx);
+    ctx->prms = *prms;
+    fill_kparams(ctx);
+    const long long nb = (long long)ctx->kp.nbx * ctx->kp.nby * ctx->kp.nbz;
+    if (nb * kBrickNodes > (1ll << 40) || nb > 0x7FFFFFF)
+        return fail(ICEMPM_ERR_INVALID_ARG, "grid too large");
+    ctx->nbricks = nb;
+    ctx->capacity = n_points_capacity;
+    ctx->pitch = ((long long)std::max(n_points_capacity, 1) + 63) / 64 * 64;
+    const size_t cap = (size_t)std::max(n_points_capacity, 1);
+    for (int i = 0; i < 2; i++) {
+        ICEMPM_TRY(dev_alloc(&ctx->pts[i], (size_t)ctx->pitch * ICEMPM_N_PTS_ARRAYS, "points"));
+        ICEMPM_TRY(dev_alloc(&ctx->slot[i], cap, "slots"));
+        ICEMPM_TRY(dev_alloc(&ctx->keys[i], cap, "keys"));
+        ICEMPM_TRY(dev_alloc(&ctx->vals[i], cap, "values"));
+        ICEMPM_TRY(dev_alloc(&ctx->brick_list[i], (size_t)nb, "brick list"));
+    }
+    size_t scr = std::max(radix_sort_scratch_ints((int)cap), 2 * cap + 1 + exclusive_scan_scratch_ints((int)cap));
+    ICEMPM_TRY(dev_alloc(&ctx->scratch, scr, "sort scratch"));
+    ICEMPM_TRY(dev_alloc(&ctx->seg_start, cap + 1, "segments"));
+    ICEMPM_TRY(dev_alloc(&ctx->seg_brick, cap, "segment bricks"));
+    ICEMPM_TRY(dev_alloc(&ctx->nseg, 1, "segment count"));
+    ICEMPM_TRY(dev_alloc(&ctx->work, cap + 1, "crushed worklist"));
+    ICEMPM_TRY(dev_alloc(&ctx->grid, (size_t)ctx->kp.grid_pitch * 4, "grid"));
+    ICEMPM_TRY(dev_alloc(&ctx->brick_flag, (size_t)nb, "brick flags"));
+    ICEMPM_TRY(dev_alloc(&ctx->brick_count, 2, "brick counts"));
+    ctx->nbins = (long long)prms->IndenterSubdivisions * prms->GridZ * 3;
+    ICEMPM_TRY(dev_alloc(&ctx->bins, (size_t)ctx->nbins, "indenter bins"));
+    ICEMPM_TRY(dev_alloc(&ctx->err, 1, "error indicator"));
+    ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(), nb, ctx->stream));
+    ICEMPM_TRY(zero_bins(ctx));
+    ICEMPM_TRY(clear_err(ctx));
+    k_fill_int<<<1, 1, 0, ctx->stream>>>(ctx->nseg, 1, 0);
+    ICEMPM_HIP(hipGetLastError());
+    k_fill_int<<<1, 1, 0, ctx->stream>>>(ctx->seg_start, 1, 0);
+    ICEMPM_HIP(hipGetLastError());
+    ctx->cur_list = 0;
+    ctx->cur_pts = 0;
+    ctx->n = 0;
+    ctx->allocated = true;
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
+}
+
+int icempm_update_constants(icempm_ctx *ctx, const icempm_SimParams3D *prms) {
+    if (!ctx || !prms) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (!ctx->allocated) return fail(ICEMPM_ERR_NOT_READY, "icempm_allocate has not been called");
+    ICEMPM_TRY(set_device(ctx));
+    const icempm_SimParams3D &old = ctx->prms;
+    if (prms->GridXTotal != old.GridXTotal || prms->GridY != old.GridY || prms->GridZ != old.GridZ ||
+        (long long)prms->IndenterSubdivisions * prms->GridZ * 3 != ctx->nbins)
+        return fail(ICEMPM_ERR_INVALID_ARG, "grid or indenter sizes differ from the allocation");
+    ctx->prms = *prms;
+    fill_kparams(ctx);
+    ctx->have_params = true;
+    return clear_err(ctx);
+}
+
+int icempm_set_indenter(icempm_ctx *ctx, const icempm_Indenter *ind) {
+    if (!ctx || !ind) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (ind->shape != ICEMPM_INDENTER_CYLINDER_Z && ind->shape != ICEMPM_INDENTER_SPHERE)
+        return fail(ICEMPM_ERR_INVALID_ARG, "unknown indenter shape");
+    ctx->ind = *ind;
+    ctx->ind_custom = true;
+    fill_kparams(ctx);
+    return ICEMPM_OK;
+}
+
+int icempm_transfer_points_to_device(icempm_ctx *ctx, const double *hssoa, unsigned capacity, unsigned offset,
+                                     int n_points) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (n_points < 0 || (n_points > 0 && !hssoa)) return fail(ICEMPM_ERR_INVALID_ARG, "bad host buffer");
+    if (n_points > ctx->capacity) return fail(ICEMPM_ERR_CAPACITY, "more points than the device capacity");
+    if ((unsigned long long)offset + (unsigned)n_points > capacity)
+        return fail(ICEMPM_ERR_CAPACITY, "offset + n_points exceeds the host capacity");
+    const int cur = ctx->cur_pts;
+    for (int l = 0; l < ICEMPM_N_PTS_ARRAYS && n_points > 0; l++) {
+        ICEMPM_HIP(hipMemcpyAsync(ctx->pts[cur] + (size_t)l * ctx->pitch, hssoa + (size_t)l * capacity + offset,
+                                  sizeof(double) * n_points, hipMemcpyHostToDevice, ctx->stream));
+    }
+    ICEMPM_HIP(launch_iota(ctx->slot[cur], n_points, ctx->stream));
+    ctx->n = n_points;
+    ICEMPM_TRY(brick_sort(ctx));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
+}
+
+int icempm_transfer_from_device(icempm_ctx *ctx, double *hssoa, unsigned capacity, unsigned offset,
+                                int *n_points_out, double *indenter_bins) {
+    ICEMPM_TRY(check_ready(ctx));
+    const int n = ctx->n;
+    if (n > 0 && !hssoa) return fail(ICEMPM_ERR_INVALID_ARG, "null host buffer");
+    if ((unsigned long long)offset + (unsigned)n > capacity)
+        return fail(ICEMPM_ERR_CAPACITY, "transfer_from_device: host capacity exceeded");
+    const int cur = ctx->cur_pts, spare = cur ^ 1;
+    ICEMPM_HIP(launch_unpermute(ctx->view(cur), ctx->view(spare), ctx->slot[cur], n, ctx->stream));
+    for (int l = 0; l < ICEMPM_N_PTS_ARRAYS && n > 0; l++) {
+        ICEMPM_HIP(hipMemcpyAsync(hssoa + (size_t)l * capacity + offset, ctx->pts[spare] + (size_t)l * ctx->pitch,
+                                  sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (indenter_bins && ctx->nbins > 0)
+        ICEMPM_HIP(hipMemcpyAsync(indenter_bins, ctx->bins, sizeof(double) * ctx->nbins, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    int code = 0;
+    ICEMPM_HIP(hipMemcpyAsync(&code, ctx->err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    if (n_points_out) *n_points_out = n;
+    if (code) return fail(ICEMPM_ERR_DEVICE_FLAG, "device error indicator " + std::to_string(code));
+    return ICEMPM_OK;
+}
+
+int icempm_reset_indenter_force_accumulator(icempm_ctx *ctx) {
+    ICEMPM_TRY(check_ready(ctx));
+    return zero_bins(ctx);
+}
+
+int icempm_reset_grid(icempm_ctx *ctx) {
+    ICEMPM_TRY(check_ready(ctx));
+    return do_reset_grid(ctx);
+}
+
+int icempm_p2g(icempm_ctx *ctx) {
+    ICEMPM_TRY(check_ready(ctx));
+    return do_p2g(ctx);
+}
+
+int icempm_update_nodes(icempm_ctx *ctx, double cx, double cy, double cz) {
+    ICEMPM_TRY(check_ready(ctx));
+    return do_update(ctx, cx, cy, cz);
+}
+
+int icempm_g2p(icempm_ctx *ctx, int record_pq) {
+    ICEMPM_TRY(check_ready(ctx));
+    return do_g2p(ctx, record_pq);
+}
+
+int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int simulation_step) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (!simulation_time || n_substeps < 0) return fail(ICEMPM_ERR_INVALID_ARG, "bad step arguments");
+    const int every = ctx->prms.UpdateEveryNthStep;
+    if (every <= 0) return fail(ICEMPM_ERR_INVALID_ARG, "UpdateEveryNthStep must be positive");
+    double t = *simulation_time;
+    for (int s = 0; s < n_substeps; s++) {
+        if (ctx->sort_period > 0 && ctx->substeps_since_sort >= ctx->sort_period) ICEMPM_TRY(brick_sort(ctx));
+        t += ctx->prms.InitialTimeStep;
+        double c[3];
+        for (int d = 0; d < 3; d++) c[d] = ctx->ind.center_initial[d] + t * ctx->ind.velocity[d];
+        ICEMPM_TRY(do_reset_grid(ctx));
+        ICEMPM_TRY(do_p2g(ctx));
+        ICEMPM_TRY(do_update(ctx, c[0], c[1], c[2]));
+        ICEMPM_TRY(do_g2p(ctx, ((simulation_step + s) % every) == 0 ? 1 : 0));
+    }
+    *simulation_time = t;
+    return ICEMPM_OK;
+}
+
+int icempm_synchronize(icempm_ctx *ctx) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    ICEMPM_TRY(set_device(ctx));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
+}
+
+int icempm_error_code(icempm_ctx *ctx, int *code) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (!code) return fail(ICEMPM_ERR_INVALID_ARG, "null code");
+    int c = 0;
+    ICEMPM_HIP(hipMemcpyAsync(&c, ctx->err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    *code = c;
+    return ICEMPM_OK;
+}
+
+int icempm_point_count(icempm_ctx *ctx, int *n) {
+    if (!ctx || !n) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    *n = ctx->n;
+    return ICEMPM_OK;
+}
+
+int icempm_remove_disabled_and_sort(icempm_ctx *ctx, int *n_points_out) {
+    ICEMPM_TRY(check_ready(ctx));
+    const int n = ctx->n;
+    if (n > 0) {
+        PhaseTimer tm(ctx, ICEMPM_PHASE_SORT);
+        ICEMPM_HIP(launch_cell_keys(ctx->kp, ctx->view(ctx->cur_pts), n, 0, ctx->keys[0], ctx->vals[0], ctx->stream));
+        ICEMPM_HIP(radix_sort_pairs(ctx->keys[0], ctx->vals[0], ctx->keys[1], ctx->vals[1], n, ctx->scratch,
+                                    ctx->stream));
+        const int nxt = ctx->cur_pts ^ 1;
+        ICEMPM_HIP(launch_permute(ctx->view(ctx->cur_pts), ctx->view(nxt), ctx->slot[ctx->cur_pts], ctx->slot[nxt],
+                                  ctx->vals[0], n, ctx->stream));
+        ctx->cur_pts = nxt;
+        ICEMPM_HIP(launch_count_valid(ctx->keys[0], n, ctx->nseg, ctx->stream));
+        int n_new = 0;
+        ICEMPM_HIP(hipMemcpyAsync(&n_new, ctx->nseg, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+        ICEMPM_TRY(tm.finish());
+        ctx->n = n_new;
+        // the sorted set becomes the upload order
+        ICEMPM_HIP(launch_iota(ctx->slot[ctx->cur_pts], ctx->n, ctx->stream));
+        ICEMPM_TRY(brick_sort(ctx));
+    }
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    if (n_points_out) *n_points_out = ctx->n;
+    return ICEMPM_OK;
+}
+
+int icempm_sort_by_brick(icempm_ctx *ctx) {
+    ICEMPM_TRY(check_ready(ctx));
+    return brick_sort(ctx);
+}
+
+int icempm_set_sort_period(icempm_ctx *ctx, int period) {
+    if (!ctx || period < 0) return fail(ICEMPM_ERR_INVALID_ARG, "bad sort period");
+    ctx->sort_period = period;
+    return ICEMPM_OK;
+}
+
+int icempm_cell_keys(icempm_ctx *ctx, int kind, uint32_t *keys_out) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (kind != 0 && kind != 1) return fail(ICEMPM_ERR_INVALID_ARG, "kind must be 0 or 1");
+    if (ctx->n > 0 && !keys_out) return fail(ICEMPM_ERR_INVALID_ARG, "null output");
+    if (ctx->n == 0) return ICEMPM_OK;
+    // keys[1] is free outside a sort
+    ICEMPM_HIP(launch_cell_keys(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, kind, ctx->keys[1], nullptr, ctx->stream));
+    ICEMPM_HIP(hipMemcpyAsync(keys_out, ctx->keys[1], sizeof(uint32_t) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
+}
+
+int icempm_device_order(icempm_ctx *ctx, int32_t *slot_out) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (ctx->n > 0 && !slot_out) return fail(ICEMPM_ERR_INVALID_ARG, "null output");
+    if (ctx->n == 0) return ICEMPM_OK;
+    ICEMPM_HIP(hipMemcpyAsync(slot_out, ctx->slot[ctx->cur_pts], sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
+}
+
+int icempm_radix_sort_pairs(icempm_ctx *ctx, const uint32_t *keys, int n, uint32_t *keys_sorted_out,
+                            int32_t *perm_out) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    if (n < 0 || (n > 0 && (!keys || !keys_sorted_out || !perm_out)))
+        return fail(ICEMPM_ERR_INVALID_ARG, "bad arguments");
+    ICEMPM_TRY(set_device(ctx));
+    if (n == 0) return ICEMPM_OK;
+    uint32_t *k0 = nullptr, *k1 = nullptr;
+    int32_t *v0 = nullptr, *v1 = nullptr, *scr = nullptr;
+    int rc = ICEMPM_OK;
+    auto cleanup = [&]() {
+        (void)hipFree(k0);
+        (void)hipFree(k1);
+        (void)hipFree(v0);
+        (void)hipFree(v1);
+        (void)hipFree(scr);
+    };
+    if ((rc = dev_alloc(&k0, n, "keys")) || (rc = dev_alloc(&k1, n, "keys")) || (rc = dev_alloc(&v0, n, "vals")) ||
+        (rc = dev_alloc(&v1, n, "vals")) || (rc = dev_alloc(&scr, radix_sort_scratch_ints(n), "scratch"))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(k0, keys, sizeof(uint32_t) * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_iota(v0, n, ctx->stream);
+    if (e == hipSuccess) e = radix_sort_pairs(k0, v0, k1, v1, n, scr, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(keys_sorted_out, k0, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(perm_out, v0, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(ICEMPM_ERR_HIP, std::string("radix_sort_pairs: ") + hipGetErrorString(e));
+    return ICEMPM_OK;
+}
+
+int icempm_download_grid(icempm_ctx *ctx, double *out) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (!out) return fail(ICEMPM_ERR_INVALID_ARG, "null output");
+    const long long nn = (long long)ctx->kp.gx * ctx->kp.gy * ctx->kp.gz;
+    if (!ctx->grid_xmajor) ICEMPM_TRY(dev_alloc(&ctx->grid_xmajor, (size_t)nn * 4, "grid copy"));
+    ICEMPM_HIP(launch_grid_to_xmajor(ctx->kp, ctx->grid, ctx->grid_xmajor, ctx->stream));
+    ICEMPM_HIP(hipMemcpyAsync(out, ctx->grid_xmajor, sizeof(double) * nn * 4, hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
+}
+
+int icempm_enable_timing(icempm_ctx *ctx, int on) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    ctx->timing = on != 0;
+    return ICEMPM_OK;
+}
+
+int icempm_reset_timings(icempm_ctx *ctx) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    ICEMPM_TRY(set_device(ctx));
+    ICEMPM_TRY(resolve_timings(ctx));
+    for (int i = 0; i < ICEMPM_N_PHASES; i++) {
+        ctx->ms[i] = 0;
+        ctx->launches[i] = 0;
+    }
+    return ICEMPM_OK;
+}
+
+int icempm_read_timings(icempm_ctx *ctx, double *ms_out, long long *launches_out) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    ICEMPM_TRY(set_device(ctx));
+    ICEMPM_TRY(resolve_timings(ctx));
+    for (int i = 0; i < ICEMPM_N_PHASES; i++) {
+        if (ms_out) ms_out[i] = ctx->ms[i];
+        if (launches_out) launches_out[i] = ctx->launches[i];
+    }
+    return ICEMPM_OK;
+}
+
+} // extern "C"

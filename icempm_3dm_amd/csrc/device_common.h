@@ -1,0 +1,108 @@
+// device_common.h — kernel-side parameter block, grid/brick addressing and
+// small FP64 helpers shared by the HIP kernels of the MPM step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "icempm_3dm.h"
+
+namespace icempm {
+
+// Grid nodes are stored in bricks of 4x4x4 nodes. Brick (bx,by,bz) has id
+// (bx*nby + by)*nbz + bz; node (x,y,z) sits at brick_id*64 + ((x&3)*4 + (y&3))*4 + (z&3)
+// of each of the 4 grid lines (m, vx, vy, vz; line l at grid + l*grid_pitch).
+constexpr int kBrick = 4;
+constexpr int kBrickNodes = 64;
+
+// Kernel-argument view of SimParams3D plus the indenter law. Passed by value
+// so that every context carries its own constants (the reference keeps one
+// __constant__ SimParams3D per device, mpm_kernels.cuh:18).
+struct KParams {
+    int gx, gy, gz;    // GridXTotal, GridY, GridZ (nodes)
+    int nbx, nby, nbz; // bricks per axis
+    long long grid_pitch;
+
+    double h, h_inv, dt;
+    double mass_solid, mass_liquid;
+    double dt_vol_Dpinv, dt_gravity, vmax, vmax_sq;
+    double mu, kappa;
+    double ice_comp, ice_tens, ice_shear, ice_tens2, nacc_beta;
+    double tan_phi, dp_threshold_p, grain_var;
+    double drag;
+    int setup_type;
+
+    int ind_shape, ind_subdiv;
+    double ind_rsq;
+    double ind_vel[3];
+};
+
+// Particle storage: 28 SoA lines of `pitch` doubles (HostSideSOA line order).
+struct PtsView {
+    double *lines; // line k at lines + k*pitch
+    long long pitch;
+    __device__ __forceinline__ double &at(int line, int i) const { return lines[(long long)line * pitch + i]; }
+};
+
+__device__ __forceinline__ long long node_index(const KParams &kp, int x, int y, int z) {
+    long long b = ((long long)(x >> 2) * kp.nby + (y >> 2)) * kp.nbz + (z >> 2);
+    return b * kBrickNodes + (((x & 3) << 4) | ((y & 3) << 2) | (z & 3));
+}
+
+__device__ __forceinline__ long long brick_of(const KParams &kp, int x, int y, int z) {
+    return ((long long)(x >> 2) * kp.nby + (y >> 2)) * kp.nbz + (z >> 2);
+}
+
+__device__ __forceinline__ double bits_to_double(long long v) { return __longlong_as_double(v); }
+__device__ __forceinline__ long long double_to_bits(double d) { return __double_as_longlong(d); }
+
+// Device error indicator: keeps the largest code raised.
+__device__ __forceinline__ void raise_error(int *err, int code) { atomicMax(err, code); }
+
+// Base node, fractional offset and error code for a particle position
+// (mpm_kernels.cuh:65-75). Returns 0 if the 3x3x3 stencil lies in the grid.
+__device__ __forceinline__ int locate(const KParams &kp, const double pos[3], int base[3], double fx[3]) {
+    double xi[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        xi[d] = pos[d] * kp.h_inv;
+        if (!(fabs(xi[d]) <= 1e9)) return ICEMPM_DEVERR_NONFINITE; // also NaN
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        base[d] = (int)(xi[d] - 0.5);
+        fx[d] = xi[d] - (double)base[d];
+    }
+    int code = 0;
+    const int g[3] = {kp.gx, kp.gy, kp.gz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (base[d] < 0) code = max(code, 70 + d);
+        if (base[d] > g[d] - 3) code = max(code, 73 + d);
+    }
+    return code;
+}
+
+// Quadratic B-spline weights of one axis (mpm_kernels.cuh:78-81).
+__device__ __forceinline__ void bspline3(double f, double w[3]) {
+    double v0 = 1.5 - f, v1 = f - 1.0, v2 = f - 0.5;
+    w[0] = 0.5 * v0 * v0;
+    w[1] = 0.75 - v1 * v1;
+    w[2] = 0.5 * v2 * v2;
+}
+
+// 3x3 determinant in Eigen's order (cofactor expansion along row 0).
+__device__ __forceinline__ double det3(const double m[9]) {
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+           m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+// Maps a linear block index so that each of the 8 XCDs (which receive
+// blocks round-robin) works on one contiguous range of logical blocks.
+// nblocks must be a multiple of 8 (launchers round grids up).
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_remap(int block, int nblocks) {
+    return (block & (kXcd - 1)) * (nblocks / kXcd) + (block / kXcd);
+}
+
+} // namespace icempm

@@ -1,0 +1,71 @@
+// kernels.h — host-side launchers of the HIP kernels (one definition per
+// kernels_*.hip file). All launchers enqueue on `stream` and return the
+// hipError_t of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.h"
+
+namespace icempm {
+
+// ---- sort (kernels_sort.hip) ----------------------------------------------
+// Key of every device point: kind 0 reference nearest-node key
+// (proxypoint3d.cpp:158-163), kind 1 brick key. Disabled points: 0xFFFFFFFF.
+hipError_t launch_cell_keys(const KParams &kp, PtsView pts, int n, int kind, uint32_t *keys,
+                            int32_t *vals, hipStream_t stream);
+// Scratch words needed by radix_sort_pairs for n items.
+size_t radix_sort_scratch_ints(int n);
+// Stable LSD radix sort (8-bit digits, 4 passes) of (keys, vals); the result
+// ends in keys/vals again (ping-pong through keys_tmp/vals_tmp).
+hipError_t radix_sort_pairs(uint32_t *keys, int32_t *vals, uint32_t *keys_tmp, int32_t *vals_tmp,
+                            int n, int32_t *scratch, hipStream_t stream);
+// out line k [i] = in line k [perm[i]] for the 28 lines, slot_out[i] = slot_in[perm[i]].
+hipError_t launch_permute(PtsView in, PtsView out, const int32_t *slot_in, int32_t *slot_out,
+                          const int32_t *perm, int n, hipStream_t stream);
+// out line k [slot[i]] = in line k [i] (back to upload order).
+hipError_t launch_unpermute(PtsView in, PtsView out, const int32_t *slot, int n, hipStream_t stream);
+// Brick segments of sorted brick keys: seg_start[s] (s < nseg) first point of
+// the s-th non-empty brick, seg_start[nseg] = number of keyed points,
+// seg_brick[s] its brick id; *nseg_out on the device.
+hipError_t launch_brick_segments(const uint32_t *sorted_keys, int n, int32_t *seg_start,
+                                 int32_t *seg_brick, int32_t *nseg_out, int32_t *scratch,
+                                 hipStream_t stream);
+// *count_out = number of keys != 0xFFFFFFFF in a sorted key array.
+hipError_t launch_count_valid(const uint32_t *sorted_keys, int n, int32_t *count_out, hipStream_t stream);
+hipError_t launch_iota(int32_t *v, int n, hipStream_t stream);
+// Device-wide exclusive prefix sum of n int32 (in place allowed).
+hipError_t exclusive_scan(const int32_t *in, int32_t *out, int n, int32_t *scratch, hipStream_t stream);
+size_t exclusive_scan_scratch_ints(int n);
+
+// ---- grid (kernels_grid.hip) ------------------------------------------------
+struct GridState {
+    double *grid;          // 4 lines of grid_pitch doubles
+    int32_t *brick_flag;   // nbricks flags (1 = in a list)
+    int32_t *brick_list[2];
+    int32_t *brick_count;  // [2]
+};
+// Zeroes the bricks of list `cur`, clears their flags, zeroes count[cur^1].
+hipError_t launch_reset_grid(const KParams &kp, GridState gs, int cur, hipStream_t stream);
+// Zeroes the whole grid, all flags and both counts (after allocation).
+hipError_t launch_clear_grid_all(const KParams &kp, GridState gs, long long nbricks, hipStream_t stream);
+// partition_kernel_update_nodes over the bricks of list `cur`.
+hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double cx, double cy, double cz,
+                               double *bins, hipStream_t stream);
+// Copy into the reference's x-major node order.
+hipError_t launch_grid_to_xmajor(const KParams &kp, const double *grid, double *out, hipStream_t stream);
+
+// ---- particle/grid transfers (kernels_p2g.hip, kernels_g2p.hip) ------------
+struct SegInfo {
+    const int32_t *seg_start; // nseg+1
+    const int32_t *seg_brick;
+    const int32_t *nseg;      // device int
+};
+hipError_t launch_p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, GridState gs, int cur,
+                      int *err, int n_blocks, hipStream_t stream);
+// G2P plus the return mapping of crushed particles (work: n ints, work_count: 1 int).
+hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, const double *grid, int record_pq, int *err, int *work,
+                      int *work_count, hipStream_t stream);
+
+} // namespace icempm

@@ -1,0 +1,166 @@
+// kernels_grid.hip — grid reset and grid update over the bricks that P2G
+// touched, and a copy of the grid into the reference's x-major node order.
+//
+// The reference clears the whole pitched grid with a memset and updates
+// every allocated node (gpu_partition_3d.cu:222-255). Here P2G appends every
+// brick it writes to a list; reset and update walk only that list, one
+// 64-lane wave per 4x4x4-node brick.
+#include "kernels.h"
+
+namespace icempm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / 64;
+
+__global__ __launch_bounds__(kThreads) void k_reset_grid(KParams kp, GridState gs, int cur) {
+    const int count = gs.brick_count[cur];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int32_t *list = gs.brick_list[cur];
+    for (int k = blockIdx.x * kWavesPerBlock + wave; k < count; k += gridDim.x * kWavesPerBlock) {
+        int b = list[k];
+        long long idx = (long long)b * kBrickNodes + lane;
+#pragma unroll
+        for (int l = 0; l < 4; l++) gs.grid[l * kp.grid_pitch + idx] = 0.0;
+        if (lane == 0) gs.brick_flag[b] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) gs.brick_count[cur ^ 1] = 0;
+}
+
+__global__ void k_clear_all(KParams kp, GridState gs, long long nbricks) {
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nbricks * kBrickNodes; i += stride) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) gs.grid[l * kp.grid_pitch + i] = 0.0;
+        if (i < nbricks) gs.brick_flag[i] = 0;
+        if (i < 2) gs.brick_count[i] = 0;
+    }
+}
+
+// partition_kernel_update_nodes (mpm_kernels.cuh:138-232) for the nodes of
+// the listed bricks; the indenter is the reference cylinder along z or a sphere.
+__global__ __launch_bounds__(kThreads) void k_update_nodes(KParams kp, GridState gs, int cur, double cx,
+                                                           double cy, double cz, double *bins) {
+    const int count = gs.brick_count[cur];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int32_t *list = gs.brick_list[cur];
+    const double dt = kp.dt;
+    const double cellsize = kp.h;
+    const bool sphere = kp.ind_shape == ICEMPM_INDENTER_SPHERE;
+    for (int k = blockIdx.x * kWavesPerBlock + wave; k < count; k += gridDim.x * kWavesPerBlock) {
+        const int b = list[k];
+        const long long idx = (long long)b * kBrickNodes + lane;
+        const double mass = gs.grid[idx];
+        if (mass == 0) continue;
+        const int bz = b % kp.nbz;
+        const int by = (b / kp.nbz) % kp.nby;
+        const int bx = b / (kp.nbz * kp.nby);
+        const int gi[3] = {bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
+        const double gnpos[3] = {gi[0] * cellsize, gi[1] * cellsize, gi[2] * cellsize};
+
+        double v[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) v[i] = gs.grid[(i + 1) * kp.grid_pitch + idx];
+#pragma unroll
+        for (int i = 0; i < 3; i++) v[i] /= mass;
+        v[1] -= kp.dt_gravity;
+        double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        if (vsq > kp.vmax_sq) {
+            double nrm = sqrt(vsq);
+#pragma unroll
+            for (int i = 0; i < 3; i++) v[i] = v[i] / nrm * kp.vmax;
+        }
+
+        if (kp.setup_type != 2) {
+            double n[3] = {gnpos[0] - cx, gnpos[1] - cy, sphere ? gnpos[2] - cz : 0.0};
+            double nsq = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+            if (nsq < kp.ind_rsq) {
+                double vrel[3] = {v[0] - kp.ind_vel[0], v[1] - kp.ind_vel[1], v[2] - kp.ind_vel[2]};
+                if (nsq > 0) {
+                    double nrm = sqrt(nsq);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) n[i] /= nrm;
+                }
+                double vn = vrel[0] * n[0] + vrel[1] * n[1] + vrel[2] * n[2];
+                if (vn < 0) {
+                    double prev[3] = {v[0], v[1], v[2]};
+#pragma unroll
+                    for (int i = 0; i < 3; i++) v[i] = kp.ind_vel[i] + (vrel[i] - n[i] * vn);
+                    double force[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) force[i] = (prev[i] - v[i]) * mass / dt;
+                    // no sensor bins to write when IndenterSubdivisions is 0
+                    if (kp.ind_subdiv > 0) {
+                    float angle = atan2f((float)n[0], (float)n[1]);
+                    angle += ICEMPM_PI;
+                    angle *= kp.ind_subdiv / (2 * ICEMPM_PI);
+                    int index_angle = min(max((int)angle, 0), kp.ind_subdiv - 1);
+                    int index_z = min(max(gi[2], 0), kp.gz - 1);
+                    int index = index_z + index_angle * kp.gz;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) unsafeAtomicAdd(&bins[i + 3 * index], force[i]);
+                    }
+                }
+            }
+        }
+
+        if (kp.setup_type != 2) {
+            if (gi[1] <= 2) v[0] = v[1] = v[2] = 0;
+        } else {
+            if (gi[1] <= 2 && v[1] < 0) v[1] = 0;
+        }
+        if (gi[1] >= kp.gy - 3 && v[1] > 0) v[1] = 0;
+        if (gi[0] <= 2 && v[0] < 0) v[0] = 0;
+        else if (gi[0] >= kp.gx - 3 && v[0] > 0) v[0] = 0;
+        if (gi[2] <= 2 && v[2] < 0) v[2] = 0;
+        else if (gi[2] >= kp.gz - 3 && v[2] > 0) v[2] = 0;
+
+        if (kp.setup_type == 2 && kp.drag != 0) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) v[i] *= (1 - kp.drag);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) gs.grid[(i + 1) * kp.grid_pitch + idx] = v[i];
+    }
+}
+
+__global__ void k_grid_to_xmajor(KParams kp, const double *grid, double *out) {
+    const long long nn = (long long)kp.gx * kp.gy * kp.gz;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += stride) {
+        int z = (int)(i % kp.gz);
+        int y = (int)((i / kp.gz) % kp.gy);
+        int x = (int)(i / ((long long)kp.gz * kp.gy));
+        long long src = node_index(kp, x, y, z);
+#pragma unroll
+        for (int l = 0; l < 4; l++) out[l * nn + i] = grid[l * kp.grid_pitch + src];
+    }
+}
+
+} // namespace
+
+hipError_t launch_reset_grid(const KParams &kp, GridState gs, int cur, hipStream_t stream) {
+    k_reset_grid<<<1024, kThreads, 0, stream>>>(kp, gs, cur);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_grid_all(const KParams &kp, GridState gs, long long nbricks, hipStream_t stream) {
+    k_clear_all<<<2048, 256, 0, stream>>>(kp, gs, nbricks);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double cx, double cy, double cz,
+                               double *bins, hipStream_t stream) {
+    k_update_nodes<<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, bins);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_to_xmajor(const KParams &kp, const double *grid, double *out, hipStream_t stream) {
+    k_grid_to_xmajor<<<2048, 256, 0, stream>>>(kp, grid, out);
+    return hipGetLastError();
+}
+
+} // namespace icempm

@@ -1,0 +1,38 @@
+"""Model3D mirror (simulation/model_3d.{h,cpp}) for one partition.
+
+``Step`` runs one cycle of ``UpdateEveryNthStep`` substeps on the device
+(model_3d.cpp:35-51, without a host synchronization per substep), then
+copies the points and indenter accumulators back and evaluates the cycle
+force (model_3d.cpp:53-60, gpu_implementation6.cpp:417-510). The
+reference's logging, snapshot mutexes and multi-partition rebalance are not
+part of the step and are left out.
+"""
+from __future__ import annotations
+
+from .capi import SimParams3D
+from .gpu import GPUImplementation6
+
+
+class Model3D:
+    def __init__(self, prms: SimParams3D, device: int = 0):
+        self.prms = prms
+        self.gpu = GPUImplementation6(device)
+        self.gpu.prms = prms
+        self.force_history: list[tuple[float, float, float, float]] = []
+
+    def Prepare(self) -> None:
+        """model_3d.cpp:132-137."""
+        self.gpu.update_constants(self.prms)
+
+    def Step(self) -> bool:
+        """One cycle; returns SimulationTime < SimulationEndTime (model_3d.cpp:111)."""
+        p = self.prms
+        self.gpu.reset_indenter_force_accumulator()
+        n = p.UpdateEveryNthStep - (p.SimulationStep % p.UpdateEveryNthStep)
+        t = self.gpu.step(n, p.SimulationTime, p.SimulationStep)
+        p.SimulationTime = t
+        p.SimulationStep += n
+        self.gpu.transfer_from_device()
+        f = self.gpu.indenter_force
+        self.force_history.append((t, float((f ** 2).sum() ** 0.5), float(f[0]), float(f[1])))
+        return p.SimulationTime < p.SimulationEndTime

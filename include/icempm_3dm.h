@@ -1,0 +1,200 @@
+/*
+ * icempm_3dm.h — C-ABI of the MI355X single-partition 3D MPM step.
+ *
+ * The library replaces the one-partition slice of the reference's device
+ * runtime: GPU_Partition_3D (simulation/gpu_partition_3d.{h,cu}) driven by
+ * GPU_Implementation6 (simulation/gpu_implementation6.{h,cpp}) and the
+ * substep loop of Model3D::Step (simulation/model_3d.cpp:35-51). Each entry
+ * point below names the reference member it stands in for.
+ *
+ * Conventions
+ *  - Every entry point returns ICEMPM_OK (0) or a positive ICEMPM_ERR_* code;
+ *    icempm_last_error() then returns a message (thread-local). Where the
+ *    reference throws std::runtime_error, this ABI returns an error code; the
+ *    C++ wrapper in icempm_gpu_implementation6.hpp turns it back into a throw.
+ *  - Host buffers are HostSideSOA buffers (host_side_soa.h:51-70): 28 lines
+ *    of `capacity` doubles, line k at base + k*capacity. The library never
+ *    keeps a host pointer beyond a call.
+ *  - A context is driven by one host thread at a time; distinct contexts are
+ *    independent. All work is issued on the context's own HIP stream, in
+ *    order; calls that return host data synchronize that stream.
+ *  - Points come back in the order they were uploaded (HostSideSOA order),
+ *    whatever order the device keeps them in.
+ */
+#ifndef ICEMPM_3DM_H
+#define ICEMPM_3DM_H
+
+#include <stdint.h>
+#include "icempm_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define ICEMPM_OK 0
+#define ICEMPM_ERR_INVALID_ARG 1   /* bad pointer, size or state */
+#define ICEMPM_ERR_HIP 2           /* a HIP runtime call failed */
+#define ICEMPM_ERR_NOT_READY 3     /* allocate/update_constants not done yet */
+#define ICEMPM_ERR_CAPACITY 4      /* host or device capacity exceeded */
+#define ICEMPM_ERR_DEVICE_FLAG 5   /* the device error indicator is set */
+#define ICEMPM_ERR_NO_DEVICE 6     /* no HIP device is visible */
+
+/* ---- device error indicator values -----------------------------------
+ * 70..75: a particle's 3x3x3 stencil leaves the grid (mpm_kernels.cuh:70-75:
+ * 70 x low, 71 y low, 72 z low, 73 x high, 74 y high, 75 z high).
+ * 76: a particle position is not finite (no reference counterpart).
+ * The indicator keeps the largest code raised since the last
+ * update_constants / clear. Offending particles are skipped, not written. */
+#define ICEMPM_DEVERR_NONFINITE 76
+
+/* ---- indenter law (extension) -----------------------------------------
+ * The reference's indenter is an infinite cylinder along z of radius
+ * sqrt(IndRSq), moving with velocity (IndVelocity, 0, 0)
+ * (mpm_kernels.cuh:154, :176-179; model_3d.cpp:38). A sphere of the same
+ * radius is added. The center at simulation time t is
+ * center_initial + t * velocity (z ignored for the cylinder). */
+#define ICEMPM_INDENTER_CYLINDER_Z 0
+#define ICEMPM_INDENTER_SPHERE 1
+typedef struct icempm_Indenter {
+    int shape;
+    double center_initial[3];
+    double velocity[3];
+} icempm_Indenter;
+
+/* Timed phases reported by icempm_read_timings (ms accumulated). */
+#define ICEMPM_PHASE_SORT 0
+#define ICEMPM_PHASE_RESET 1
+#define ICEMPM_PHASE_P2G 2
+#define ICEMPM_PHASE_UPDATE 3
+#define ICEMPM_PHASE_G2P 4
+#define ICEMPM_N_PHASES 5
+
+typedef struct icempm_ctx icempm_ctx;
+
+/* Message for the last failed call on this thread ("" if none). */
+const char *icempm_last_error(void);
+/* Library version string. */
+const char *icempm_version(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int icempm_get_device_count(int *count);
+
+/* GPU_Implementation6::initialize (gpu_implementation6.cpp:319-337) with one
+ * partition + GPU_Partition_3D::initialize (gpu_partition_3d.cu:52-73):
+ * binds a device and creates the context's stream. */
+int icempm_create(int device, icempm_ctx **out);
+/* GPU_Partition_3D::~GPU_Partition_3D (gpu_partition_3d.cu:27-50). */
+int icempm_destroy(icempm_ctx *ctx);
+
+/* GPU_Implementation6::allocate_arrays (gpu_implementation6.cpp:385-401) and
+ * GPU_Partition_3D::allocate (gpu_partition_3d.cu:76-132): device particle
+ * storage for n_points_capacity points, the GridXTotal x GridY x GridZ grid
+ * (no halo slices: one partition) and IndenterSubdivisions*GridZ*3 bins. */
+int icempm_allocate(icempm_ctx *ctx, const icempm_SimParams3D *prms, int n_points_capacity);
+
+/* GPU_Partition_3D::update_constants (gpu_partition_3d.cu:135-143): takes a
+ * copy of the parameters and clears the device error indicator. Unless
+ * icempm_set_indenter was called, the indenter law is reset to the
+ * reference cylinder: center (indenter_x_initial, indenter_y_initial, 0),
+ * velocity (IndVelocity, 0, 0). */
+int icempm_update_constants(icempm_ctx *ctx, const icempm_SimParams3D *prms);
+
+/* Extension: replace the indenter law (shape, initial center, velocity). */
+int icempm_set_indenter(icempm_ctx *ctx, const icempm_Indenter *ind);
+
+/* GPU_Partition_3D::transfer_points_from_soa_to_device
+ * (gpu_partition_3d.cu:201-215): n_points points from host lines
+ * [offset, offset+n_points) become the device's points, in that order. */
+int icempm_transfer_points_to_device(icempm_ctx *ctx, const double *hssoa, unsigned capacity,
+                                     unsigned offset, int n_points);
+
+/* GPU_Partition_3D::transfer_from_device (gpu_partition_3d.cu:169-199) plus
+ * the checks of GPU_Implementation6::transfer_from_device
+ * (gpu_implementation6.cpp:417-448): writes the points back to host lines
+ * [offset, offset+n) in upload order, copies the raw indenter accumulators
+ * (IndenterSubdivisions*GridZ*3 doubles) if indenter_bins is not NULL and
+ * reads the error indicator. Returns ICEMPM_ERR_DEVICE_FLAG (data still
+ * copied) when the indicator is set, ICEMPM_ERR_CAPACITY if
+ * offset + n > capacity. */
+int icempm_transfer_from_device(icempm_ctx *ctx, double *hssoa, unsigned capacity, unsigned offset,
+                                int *n_points_out, double *indenter_bins);
+
+/* GPU_Partition_3D::reset_indenter_force_accumulator (gpu_partition_3d.cu:145-150) */
+int icempm_reset_indenter_force_accumulator(icempm_ctx *ctx);
+/* GPU_Partition_3D::reset_grid (gpu_partition_3d.cu:222-229) */
+int icempm_reset_grid(icempm_ctx *ctx);
+/* GPU_Partition_3D::p2g (gpu_partition_3d.cu:231-241) */
+int icempm_p2g(icempm_ctx *ctx);
+/* GPU_Partition_3D::update_nodes (gpu_partition_3d.cu:243-255); the
+ * indenter center is passed explicitly, as the reference passes
+ * (indenter_x, indenter_y) as a kernel argument (cz used by the sphere). */
+int icempm_update_nodes(icempm_ctx *ctx, double cx, double cy, double cz);
+/* GPU_Partition_3D::g2p (gpu_partition_3d.cu:258-292) without point
+ * transfer (one partition). record_pq != 0 writes P and Q lines. */
+int icempm_g2p(icempm_ctx *ctx, int record_pq);
+
+/* Model3D::Step substep loop (model_3d.cpp:35-51) on the device, without a
+ * host synchronization per substep: for s in [0, n_substeps):
+ * *simulation_time += dt; center = law(*simulation_time); reset_grid; p2g;
+ * update_nodes(center); g2p((simulation_step + s) % UpdateEveryNthStep == 0).
+ * If a sort period is set, points are regrouped by grid brick first on every
+ * substep whose global index is a multiple of it. Asynchronous. */
+int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int simulation_step);
+
+/* GPU_Implementation6::synchronize (gpu_implementation6.cpp:513-520) */
+int icempm_synchronize(icempm_ctx *ctx);
+/* Reads the device error indicator (synchronizes). */
+int icempm_error_code(icempm_ctx *ctx, int *code);
+/* Number of points currently on the device. */
+int icempm_point_count(icempm_ctx *ctx, int *n);
+
+/* HostSideSOA::RemoveDisabledAndSort (host_side_soa.cpp:56-67) on the device:
+ * disabled points are dropped and the rest are ordered by the reference key
+ * ProxyPoint3D::getCellIndex (proxypoint3d.cpp:158-163) with a stable LSD
+ * radix sort (ties keep their previous order). The new order becomes the
+ * upload order, so the next transfer_from_device returns the sorted set. */
+int icempm_remove_disabled_and_sort(icempm_ctx *ctx, int *n_points_out);
+/* Regroups the device points by brick key (4^3-cell bricks, cells ordered
+ * inside a brick); upload order is preserved for transfer_from_device. */
+int icempm_sort_by_brick(icempm_ctx *ctx);
+/* Extension: icempm_step regroups by brick every `period` substeps (0: never). */
+int icempm_set_sort_period(icempm_ctx *ctx, int period);
+
+/* Keys of the device points in device order (kind 0: reference key,
+ * kind 1: brick key; disabled points get 0xFFFFFFFF), copied to keys_out. */
+int icempm_cell_keys(icempm_ctx *ctx, int kind, uint32_t *keys_out);
+/* Device order of the points: slot_out[i] = upload index of device point i. */
+int icempm_device_order(icempm_ctx *ctx, int32_t *slot_out);
+/* Stable LSD radix sort of n (key, index) pairs on the device: keys and
+ * perm_out are host arrays; keys_sorted_out/perm_out receive the result. */
+int icempm_radix_sort_pairs(icempm_ctx *ctx, const uint32_t *keys, int n, uint32_t *keys_sorted_out,
+                            int32_t *perm_out);
+/* Copies the grid out in the reference's x-major node order
+ * (x*GY*GZ + y*GZ + z), 4 lines of GX*GY*GZ doubles (m, vx|mvx, vy|mvy, vz|mvz). */
+int icempm_download_grid(icempm_ctx *ctx, double *out);
+
+/* Per-phase HIP-event timing of the kernels on the context's stream. */
+int icempm_enable_timing(icempm_ctx *ctx, int on);
+int icempm_reset_timings(icempm_ctx *ctx);
+/* ms_out[ICEMPM_N_PHASES] accumulated ms, launches_out[ICEMPM_N_PHASES]
+ * number of timed launches. Synchronizes. */
+int icempm_read_timings(icempm_ctx *ctx, double *ms_out, long long *launches_out);
+
+/* Synthetic input (no reference counterpart; replaces the HDF5 point cloud of
+ * SnapshotManager::LoadRawPoints for generated scenes): a block of
+ * cells[0]*cells[1]*cells[2] lattice cells of edge `cell_size` starting at
+ * `origin`, ppc_axis^3 points per cell on a sub-lattice of spacing
+ * cell_size/ppc_axis, each displaced by U(-jitter, +jitter)*spacing per axis
+ * and given a grain id U{0..n_grains-1}; draws come from SplitMix64 keyed by
+ * (seed, point index). Writes points [0, n) of the host SoA with
+ * Fe = I, Jp_inv = 1 and zeros elsewhere (HostSideSOA::InitializeBlock,
+ * host_side_soa.cpp:88-97). Returns the point count in *n_out. */
+int icempm_scene_block(double *hssoa, unsigned capacity, const int cells[3], int ppc_axis,
+                       const double origin[3], double cell_size, double jitter,
+                       unsigned long long seed, int n_grains, int *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ICEMPM_3DM_H */
