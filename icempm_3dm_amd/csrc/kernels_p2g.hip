@@ -7,10 +7,12 @@
 // particles that drifted since the sort. Per 256-particle sub-chunk:
 //   phase 1  one lane per particle: stress, APIC affine matrix, weights ->
 //            LDS record (22 doubles);
-//   phase 2  runs of particles sharing a base node (same cell) are taken by
-//            half-waves whose 27 lanes are the 27 stencil nodes; each lane
-//            sums its node's mass and momentum over the run in registers,
-//            then does 4 conflict-free ds_add_f64 into the tile;
+//   phase 2  the sub-chunk is cut into 28 slices, one per 9-lane group
+//            (7 groups per wave); lane (ox,oy) of a group owns the column of
+//            3 stencil nodes (ox,oy,0..2) and sums their mass and momentum
+//            over consecutive particles sharing a base node in registers,
+//            flushing to the tile with ds_add_f64 when the base changes.
+//            Each lane reads 18 doubles per particle for 3 nodes;
 //   phase 2b a particle outside the tile (drifted more than one cell since
 //            the sort) is taken by a half-wave alone and its 27 nodes go
 //            straight to HBM with FP64 atomics;
@@ -30,29 +32,51 @@ constexpr int kTile = 8;
 constexpr int kTileNodes = kTile * kTile * kTile;
 constexpr int kRec = 22; // m | wx[3] wy[3] wz[3] | a[3] | Mh[9]
 
-__device__ __forceinline__ void mark_brick(GridState gs, int cur, long long b) {
-    if (gs.brick_flag[b] == 0 && atomicCAS(&gs.brick_flag[b], 0, 1) == 0) {
-        int slot = atomicAdd(&gs.brick_count[cur], 1);
-        gs.brick_list[cur][slot] = (int32_t)b;
+// Appends brick b to the active list once (its flag guards the append).
+struct BrickList {
+    int32_t *flag;
+    int32_t *list;
+    int32_t *count;
+};
+
+__device__ __forceinline__ void mark_brick(const BrickList &bl, long long b) {
+    if (bl.flag[b] == 0 && atomicCAS(&bl.flag[b], 0, 1) == 0) {
+        int slot = atomicAdd(bl.count, 1);
+        bl.list[slot] = (int32_t)b;
     }
 }
 
-// Stencil-node contribution of one LDS particle record for the lane's
-// offset (ox,oy,oz): W * m and W * (a + Mh o).
-struct NodeSum {
-    double m, v0, v1, v2;
+// Column accumulator of one group lane: mass and momentum of the stencil
+// nodes (ox, oy, 0..2) of the current base node.
+struct Column {
+    double m[3], v0[3], v1[3], v2[3];
 };
 
-__device__ __forceinline__ void accumulate(const double (*rec)[kChunk], int j, int ox, int oy, int oz, double dox,
-                                           double doy, double doz, NodeSum &acc) {
-    const double W = rec[1 + ox][j] * rec[4 + oy][j] * rec[7 + oz][j];
-    acc.m += W * rec[0][j];
-    const double u0 = rec[10][j] + rec[13][j] * dox + rec[14][j] * doy + rec[15][j] * doz;
-    const double u1 = rec[11][j] + rec[16][j] * dox + rec[17][j] * doy + rec[18][j] * doz;
-    const double u2 = rec[12][j] + rec[19][j] * dox + rec[20][j] * doy + rec[21][j] * doz;
-    acc.v0 += W * u0;
-    acc.v1 += W * u1;
-    acc.v2 += W * u2;
+__device__ __forceinline__ void column_zero(Column &c) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) c.m[k] = c.v0[k] = c.v1[k] = c.v2[k] = 0.0;
+}
+
+// Adds particle j of the LDS records: W (m, a + Mh o) for o = (ox, oy, k).
+__device__ __forceinline__ void column_add(const double (*rec)[kChunk], int j, int ox, int oy, double dox,
+                                           double doy, Column &c) {
+    const double wxy = rec[1 + ox][j] * rec[4 + oy][j];
+    const double m = rec[0][j];
+    double base[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) base[r] = rec[10 + r][j] + rec[13 + 3 * r][j] * dox + rec[14 + 3 * r][j] * doy;
+    const double mz[3] = {rec[15][j], rec[18][j], rec[21][j]}; // Mh(r, 2)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double W = wxy * rec[7 + k][j];
+        c.m[k] += W * m;
+        const double u0 = (k == 0) ? base[0] : base[0] + mz[0] * (double)k;
+        const double u1 = (k == 0) ? base[1] : base[1] + mz[1] * (double)k;
+        const double u2 = (k == 0) ? base[2] : base[2] + mz[2] * (double)k;
+        c.v0[k] += W * u0;
+        c.v1[k] += W * u1;
+        c.v2[k] += W * u2;
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void k_p2g(KParams kp, PtsView pts, SegInfo seg, GridState gs, int cur,
@@ -61,27 +85,29 @@ __global__ __launch_bounds__(kThreads) void k_p2g(KParams kp, PtsView pts, SegIn
     __shared__ double s_rec[kRec][kChunk];
     __shared__ int s_base[kChunk];   // tile-relative base (packed), -1 none, -2 overflow
     __shared__ int s_abs[kChunk];    // absolute base of overflow particles (packed 10:10:10 bits)
-    __shared__ int s_runs[kChunk];
     __shared__ int s_over[kChunk];
-    __shared__ int s_wave_count[2][kThreads / 64];
-    __shared__ int s_nruns, s_nover;
+    __shared__ int s_wave_count[kThreads / 64];
+    __shared__ int s_nover;
     __shared__ unsigned s_brickmask;
 
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
     const int nseg = *seg.nseg;
+    // select, not index: a runtime index into the by-value struct would spill it
+    const BrickList bl{gs.brick_flag, cur ? gs.brick_list[1] : gs.brick_list[0], gs.brick_count + (cur ? 1 : 0)};
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     // a contiguous range of segments per logical block
     const long long s_lo = (long long)nseg * lb / gridDim.x;
     const long long s_hi = (long long)nseg * (lb + 1) / gridDim.x;
 
-    // phase-2 lane roles: lanes 0..26 of each half-wave are stencil offsets
-    const int hl = t & 31;
-    const int half = t >> 5; // 0..7
-    const bool node_lane = hl < 27;
-    const int ox = node_lane ? hl / 9 : 0, oy = node_lane ? (hl / 3) % 3 : 0, oz = node_lane ? hl % 3 : 0;
-    const double dox = ox, doy = oy, doz = oz;
+    // phase-2 lane roles: 7 groups of 9 lanes per wave; lane (ox,oy) owns a node column
+    const int gq = lane % 9;
+    const bool group_lane = lane < 63;
+    const int group = wave * 7 + lane / 9; // 0..27 for group lanes
+    const int ox = gq / 3, oy = gq % 3;
+    const double dox = ox, doy = oy;
+    constexpr int kGroups = (kThreads / 64) * 7;
 
     for (long long s = s_lo; s < s_hi; s++) {
         const int begin = seg.seg_start[s];
@@ -163,70 +189,83 @@ __global__ __launch_bounds__(kThreads) void k_p2g(KParams kp, PtsView pts, SegIn
             s_base[t] = packed;
             __syncthreads();
 
-            // ---------------- runs of equal base node, and the overflow list
-            const bool head = packed >= 0 && (t == 0 || s_base[t - 1] != packed);
+            // ---------------- list of particles outside the tile
             const bool over = packed == -2;
-            const unsigned long long hm = __ballot(head);
             const unsigned long long om = __ballot(over);
-            if (lane == 0) {
-                s_wave_count[0][wave] = __popcll(hm);
-                s_wave_count[1][wave] = __popcll(om);
-            }
+            if (lane == 0) s_wave_count[wave] = __popcll(om);
             __syncthreads();
             {
-                int offh = 0, offo = 0;
-                for (int w = 0; w < wave; w++) {
-                    offh += s_wave_count[0][w];
-                    offo += s_wave_count[1][w];
-                }
-                const unsigned long long below = (1ull << lane) - 1ull;
-                if (head) s_runs[offh + __popcll(hm & below)] = t;
-                if (over) s_over[offo + __popcll(om & below)] = t;
+                int offo = 0;
+                for (int w = 0; w < wave; w++) offo += s_wave_count[w];
+                if (over) s_over[offo + __popcll(om & ((1ull << lane) - 1ull))] = t;
                 if (t == 0) {
-                    int th = 0, to = 0;
-                    for (int w = 0; w < kThreads / 64; w++) {
-                        th += s_wave_count[0][w];
-                        to += s_wave_count[1][w];
-                    }
-                    s_nruns = th;
+                    int to = 0;
+                    for (int w = 0; w < kThreads / 64; w++) to += s_wave_count[w];
                     s_nover = to;
                 }
             }
             __syncthreads();
 
-            // ---------------- phase 2: half-wave per run, lane per stencil node
-            const int nruns = s_nruns;
-            for (int r = half; r < nruns; r += kThreads / 32) {
-                const int j0 = s_runs[r];
-                const int pk = s_base[j0];
-                NodeSum acc{0.0, 0.0, 0.0, 0.0};
-                for (int j = j0; j < kChunk && s_base[j] == pk; j++)
-                    accumulate(s_rec, j, ox, oy, oz, dox, doy, doz, acc);
-                if (node_lane) {
-                    const int nx = (pk & 7) + ox, ny = ((pk >> 3) & 7) + oy, nz = ((pk >> 6) & 7) + oz;
-                    const int tn = (nx * kTile + ny) * kTile + nz;
-                    atomicAdd(&s_tile[0][tn], acc.m);
-                    atomicAdd(&s_tile[1][tn], acc.v0);
-                    atomicAdd(&s_tile[2][tn], acc.v1);
-                    atomicAdd(&s_tile[3][tn], acc.v2);
+            // ---------------- phase 2: group slices, lane per node column
+            {
+                const int nchunk = min(kChunk, end - c0);
+                const int per = (nchunk + kGroups - 1) / kGroups;
+                const int j_lo = group_lane ? min(group * per, nchunk) : nchunk;
+                const int j_hi = min(j_lo + per, nchunk);
+                Column col;
+                column_zero(col);
+                int cur_pk = -1;
+                for (int j = j_lo; j < j_hi; j++) {
+                    const int pk = s_base[j];
+                    if (pk < 0) continue;
+                    if (pk != cur_pk) {
+                        if (cur_pk >= 0) {
+                            const int tn = ((((cur_pk & 7) + ox) * kTile + ((cur_pk >> 3) & 7) + oy) * kTile) + ((cur_pk >> 6) & 7);
+#pragma unroll
+                            for (int k = 0; k < 3; k++) {
+                                atomicAdd(&s_tile[0][tn + k], col.m[k]);
+                                atomicAdd(&s_tile[1][tn + k], col.v0[k]);
+                                atomicAdd(&s_tile[2][tn + k], col.v1[k]);
+                                atomicAdd(&s_tile[3][tn + k], col.v2[k]);
+                            }
+                            column_zero(col);
+                        }
+                        cur_pk = pk;
+                    }
+                    column_add(s_rec, j, ox, oy, dox, doy, col);
+                }
+                if (cur_pk >= 0) {
+                    const int tn = ((((cur_pk & 7) + ox) * kTile + ((cur_pk >> 3) & 7) + oy) * kTile) + ((cur_pk >> 6) & 7);
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        atomicAdd(&s_tile[0][tn + k], col.m[k]);
+                        atomicAdd(&s_tile[1][tn + k], col.v0[k]);
+                        atomicAdd(&s_tile[2][tn + k], col.v1[k]);
+                        atomicAdd(&s_tile[3][tn + k], col.v2[k]);
+                    }
                 }
             }
             // ---------------- phase 2b: particles outside the tile go straight to HBM
             const int nover = s_nover;
-            for (int r = half; r < nover; r += kThreads / 32) {
+            for (int r = group_lane ? group : nover; r < nover; r += kGroups) {
                 const int j = s_over[r];
                 const int ab = s_abs[j];
-                NodeSum acc{0.0, 0.0, 0.0, 0.0};
-                accumulate(s_rec, j, ox, oy, oz, dox, doy, doz, acc);
-                if (node_lane) {
-                    const int x = (ab & 1023) + ox, y = ((ab >> 10) & 1023) + oy, z = ((ab >> 20) & 1023) + oz;
-                    const long long idx = node_index(kp, x, y, z);
-                    unsafeAtomicAdd(&gs.grid[idx], acc.m);
-                    unsafeAtomicAdd(&gs.grid[kp.grid_pitch + idx], acc.v0);
-                    unsafeAtomicAdd(&gs.grid[2 * kp.grid_pitch + idx], acc.v1);
-                    unsafeAtomicAdd(&gs.grid[3 * kp.grid_pitch + idx], acc.v2);
-                    // the 8 corner offsets reach every brick the stencil touches
-                    if (ox != 1 && oy != 1 && oz != 1) mark_brick(gs, cur, brick_of(kp, x, y, z));
+                Column col;
+                column_zero(col);
+                column_add(s_rec, j, ox, oy, dox, doy, col);
+                const int x = (ab & 1023) + ox, y = ((ab >> 10) & 1023) + oy, z0 = (ab >> 20) & 1023;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const long long idx = node_index(kp, x, y, z0 + k);
+                    unsafeAtomicAdd(&gs.grid[idx], col.m[k]);
+                    unsafeAtomicAdd(&gs.grid[kp.grid_pitch + idx], col.v0[k]);
+                    unsafeAtomicAdd(&gs.grid[2 * kp.grid_pitch + idx], col.v1[k]);
+                    unsafeAtomicAdd(&gs.grid[3 * kp.grid_pitch + idx], col.v2[k]);
+                }
+                // the corner columns reach every brick the stencil touches
+                if (ox != 1 && oy != 1) {
+                    mark_brick(bl, brick_of(kp, x, y, z0));
+                    mark_brick(bl, brick_of(kp, x, y, z0 + 2));
                 }
             }
             __syncthreads();
@@ -251,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g(KParams kp, PtsView pts, SegIn
         if (t < 27 && ((s_brickmask >> t) & 1u)) {
             const int dbx = t / 9, dby = (t / 3) % 3, dbz = t % 3;
             const long long b = ((long long)(bx - 1 + dbx) * kp.nby + (by - 1 + dby)) * kp.nbz + (bz - 1 + dbz);
-            mark_brick(gs, cur, b);
+            mark_brick(bl, b);
         }
         __syncthreads();
     }

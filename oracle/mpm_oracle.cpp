@@ -167,38 +167,47 @@ void svd3(const Mat3 &F, Mat3 &U, double S[3], Mat3 &V) {
         o[2] = a[0] * b[1] - a[1] * b[0];
     };
     auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-    // V must be a rotation: fix the sign of its third column (U follows).
+    // V must be a rotation: fix the sign of its third column (the column of
+    // A = F V follows).
     double vx[3];
     cross(Vs[0], Vs[1], vx);
+    double A2[3];
+    for (int r = 0; r < 3; r++) A2[r] = ss[2] * Us[2][r];
     if (dot(vx, Vs[2]) < 0)
         for (int r = 0; r < 3; r++) {
             Vs[2][r] = -Vs[2][r];
-            Us[2][r] = -Us[2][r];
+            A2[r] = -A2[r];
         }
-    // Degenerate trailing columns of U are completed from the leading ones.
-    if (ss[1] <= 1e-300 * (ss[0] > 0 ? ss[0] : 1.0) || ss[0] == 0.0) {
-        // rank <= 1: build an orthonormal frame around U0 (or e0)
-        double u0[3] = {1, 0, 0};
-        if (ss[0] > 0) std::memcpy(u0, Us[0], sizeof(u0));
+    // U by Gram-Schmidt on the columns of A, completed where a singular value
+    // is negligible; U2 = U0 x U1 makes U a rotation and the signed third
+    // singular value is A2 . U2 (negative iff det F < 0).
+    const double tiny = 8 * DBL_EPSILON * ss[0];
+    double u0[3] = {1, 0, 0}, u1[3], u2[3];
+    if (ss[0] > 0) std::memcpy(u0, Us[0], sizeof(u0));
+    bool have_u1 = false;
+    if (ss[1] > tiny && ss[1] > 0) {
+        double a1[3], d = 0;
+        for (int r = 0; r < 3; r++) a1[r] = ss[1] * Us[1][r];
+        d = dot(u0, a1);
+        for (int r = 0; r < 3; r++) u1[r] = a1[r] - d * u0[r];
+        double nn = std::sqrt(dot(u1, u1));
+        if (nn > tiny) {
+            for (double &x : u1) x /= nn;
+            have_u1 = true;
+        }
+    }
+    if (!have_u1) {
         double e[3] = {0, 0, 0};
         e[std::fabs(u0[0]) < 0.9 ? 0 : 1] = 1.0;
-        double u1[3];
         cross(u0, e, u1);
         double nn = std::sqrt(dot(u1, u1));
         for (double &x : u1) x /= nn;
-        std::memcpy(Us[0], u0, sizeof(u0));
-        std::memcpy(Us[1], u1, sizeof(u1));
     }
-    double ux[3];
-    cross(Us[0], Us[1], ux);
-    if (ss[2] <= 1e-300 * (ss[0] > 0 ? ss[0] : 1.0)) {
-        std::memcpy(Us[2], ux, sizeof(ux));
-        ss[2] = 0.0;
-    } else if (dot(ux, Us[2]) < 0) {
-        // det F < 0: U rotation, negative smallest singular value.
-        for (int r = 0; r < 3; r++) Us[2][r] = -Us[2][r];
-        ss[2] = -ss[2];
-    }
+    cross(u0, u1, u2);
+    ss[2] = dot(A2, u2);
+    std::memcpy(Us[0], u0, sizeof(u0));
+    std::memcpy(Us[1], u1, sizeof(u1));
+    std::memcpy(Us[2], u2, sizeof(u2));
     for (int k = 0; k < 3; k++) {
         S[k] = ss[k];
         for (int r = 0; r < 3; r++) {
