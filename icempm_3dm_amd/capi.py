@@ -166,6 +166,7 @@ _PROTOS = {
     "icempm_remove_disabled_and_sort": (C.c_int, [_ctxp, _ip]),
     "icempm_sort_by_brick": (C.c_int, [_ctxp]),
     "icempm_set_sort_period": (C.c_int, [_ctxp, C.c_int]),
+    "icempm_set_fusion": (C.c_int, [_ctxp, C.c_int]),
     "icempm_cell_keys": (C.c_int, [_ctxp, C.c_int, _u32p]),
     "icempm_device_order": (C.c_int, [_ctxp, _i32p]),
     "icempm_radix_sort_pairs": (C.c_int, [_ctxp, _u32p, C.c_int, _u32p, _i32p]),

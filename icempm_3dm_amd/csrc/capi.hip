@@ -69,16 +69,22 @@ struct icempm_ctx {
     int32_t *seg_start = nullptr;
     int32_t *seg_brick = nullptr;
     int32_t *nseg = nullptr;
-    int32_t *work = nullptr; // crushed-particle worklist (+1 count)
+    int32_t *work = nullptr; // G2P worklists: 2 counts, crushed list, spill list
     int p2g_blocks = 2048;
+    int g2p_blocks = 4096;
 
-    // grid
-    double *grid = nullptr;
+    // grids: two buffers (the fused substep reads one and writes the other),
+    // each with its own active-brick flags and ping-pong lists
+    struct GridBuf {
+        double *grid = nullptr;
+        int32_t *flag = nullptr;
+        int32_t *list[2] = {nullptr, nullptr};
+        int32_t *count = nullptr;
+        int cur = 0;
+    } grids[2];
+    int g = 0; // the grid the per-phase calls use, and the last one updated
     long long nbricks = 0;
-    int32_t *brick_flag = nullptr;
-    int32_t *brick_list[2] = {nullptr, nullptr};
-    int32_t *brick_count = nullptr;
-    int cur_list = 0;
+    bool fuse = false;
     double *bins = nullptr;
     long long nbins = 0;
     int *err = nullptr;
@@ -95,14 +101,14 @@ struct icempm_ctx {
     long long launches[ICEMPM_N_PHASES] = {0};
 
     PtsView view(int which) const { return PtsView{pts[which], pitch}; }
-    GridState gstate() const {
-        GridState g;
-        g.grid = grid;
-        g.brick_flag = brick_flag;
-        g.brick_list[0] = brick_list[0];
-        g.brick_list[1] = brick_list[1];
-        g.brick_count = brick_count;
-        return g;
+    GridState gstate(int gi) const {
+        GridState gs;
+        gs.grid = grids[gi].grid;
+        gs.brick_flag = grids[gi].flag;
+        gs.brick_list[0] = grids[gi].list[0];
+        gs.brick_list[1] = grids[gi].list[1];
+        gs.brick_count = grids[gi].count;
+        return gs;
     }
 };
 
@@ -227,27 +233,27 @@ void free_all(icempm_ctx *ctx) {
         f(ctx->slot[i]);
         f(ctx->keys[i]);
         f(ctx->vals[i]);
-        f(ctx->brick_list[i]);
         ctx->pts[i] = nullptr;
         ctx->slot[i] = nullptr;
         ctx->keys[i] = nullptr;
         ctx->vals[i] = nullptr;
-        ctx->brick_list[i] = nullptr;
+        auto &gb = ctx->grids[i];
+        f(gb.grid);
+        f(gb.flag);
+        f(gb.list[0]);
+        f(gb.list[1]);
+        f(gb.count);
+        gb = icempm_ctx::GridBuf{};
     }
     f(ctx->scratch);
     f(ctx->seg_start);
     f(ctx->seg_brick);
     f(ctx->nseg);
     f(ctx->work);
-    f(ctx->grid);
-    f(ctx->brick_flag);
-    f(ctx->brick_count);
     f(ctx->bins);
     f(ctx->err);
     f(ctx->grid_xmajor);
     ctx->scratch = ctx->seg_start = ctx->seg_brick = ctx->nseg = ctx->work = nullptr;
-    ctx->grid = nullptr;
-    ctx->brick_flag = ctx->brick_count = nullptr;
     ctx->bins = nullptr;
     ctx->err = nullptr;
     ctx->grid_xmajor = nullptr;
@@ -280,33 +286,49 @@ int brick_sort(icempm_ctx *ctx) {
     return tm.finish();
 }
 
-int do_reset_grid(icempm_ctx *ctx) {
+int do_reset_grid(icempm_ctx *ctx, int gi) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_RESET);
-    ICEMPM_HIP(launch_reset_grid(ctx->kp, ctx->gstate(), ctx->cur_list, ctx->stream));
-    ctx->cur_list ^= 1;
+    auto &gb = ctx->grids[gi];
+    ICEMPM_HIP(launch_reset_grid(ctx->kp, ctx->gstate(gi), gb.cur, ctx->stream));
+    gb.cur ^= 1;
     return tm.finish();
 }
 
-int do_p2g(icempm_ctx *ctx) {
+int do_p2g(icempm_ctx *ctx, int gi) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_P2G);
     SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
-    ICEMPM_HIP(launch_p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->gstate(), ctx->cur_list, ctx->err,
+    ICEMPM_HIP(launch_p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->gstate(gi), ctx->grids[gi].cur, ctx->err,
                           ctx->p2g_blocks, ctx->stream));
     return tm.finish();
 }
 
-int do_update(icempm_ctx *ctx, double cx, double cy, double cz) {
+int do_update(icempm_ctx *ctx, int gi, double cx, double cy, double cz) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_UPDATE);
-    ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(), ctx->cur_list, cx, cy, cz, ctx->bins, ctx->stream));
+    ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(gi), ctx->grids[gi].cur, cx, cy, cz, ctx->bins, ctx->stream));
     return tm.finish();
 }
 
-int do_g2p(icempm_ctx *ctx, int record_pq) {
+int do_g2p(icempm_ctx *ctx, int gi, int record_pq) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_G2P);
-    ICEMPM_HIP(launch_g2p(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, ctx->grid, record_pq, ctx->err, ctx->work + 1,
-                          ctx->work, ctx->stream));
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    ICEMPM_HIP(launch_g2p(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[gi].grid, record_pq, ctx->err,
+                          ctx->work, ctx->g2p_blocks, ctx->stream));
     ctx->substeps_since_sort++;
     return tm.finish();
+}
+
+// G2P of one substep from grid ga fused with P2G of the next into grid gb.
+int do_g2p2g(icempm_ctx *ctx, int ga, int gb, int record_pq) {
+    PhaseTimer tm(ctx, ICEMPM_PHASE_G2P);
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    ICEMPM_HIP(launch_g2p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[ga].grid, ctx->gstate(gb),
+                            ctx->grids[gb].cur, record_pq, ctx->err, ctx->work, ctx->p2g_blocks, ctx->stream));
+    ctx->substeps_since_sort++;
+    return tm.finish();
+}
+
+void indenter_center(const icempm_ctx *ctx, double t, double c[3]) {
+    for (int d = 0; d < 3; d++) c[d] = ctx->ind.center_initial[d] + t * ctx->ind.velocity[d];
 }
 
 __global__ void k_fill_double(double *p, long long n, double v) {
@@ -365,6 +387,8 @@ int icempm_create(int device, icempm_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->p2g_blocks = prop.multiProcessorCount * 8;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->g2p_blocks = prop.multiProcessorCount * 16;
     *out = ctx;
     return ICEMPM_OK;
 }
@@ -409,28 +433,31 @@ int icempm_allocate(icempm_ctx *ctx, const icempm_SimParams3D *prms, int n_point
         ICEMPM_TRY(dev_alloc(&ctx->slot[i], cap, "slots"));
         ICEMPM_TRY(dev_alloc(&ctx->keys[i], cap, "keys"));
         ICEMPM_TRY(dev_alloc(&ctx->vals[i], cap, "values"));
-        ICEMPM_TRY(dev_alloc(&ctx->brick_list[i], (size_t)nb, "brick list"));
+        auto &gb = ctx->grids[i];
+        ICEMPM_TRY(dev_alloc(&gb.grid, (size_t)ctx->kp.grid_pitch * 4, "grid"));
+        ICEMPM_TRY(dev_alloc(&gb.flag, (size_t)nb, "brick flags"));
+        ICEMPM_TRY(dev_alloc(&gb.list[0], (size_t)nb, "brick list"));
+        ICEMPM_TRY(dev_alloc(&gb.list[1], (size_t)nb, "brick list"));
+        ICEMPM_TRY(dev_alloc(&gb.count, 2, "brick counts"));
+        gb.cur = 0;
     }
     size_t scr = std::max(radix_sort_scratch_ints((int)cap), 2 * cap + 1 + exclusive_scan_scratch_ints((int)cap));
     ICEMPM_TRY(dev_alloc(&ctx->scratch, scr, "sort scratch"));
     ICEMPM_TRY(dev_alloc(&ctx->seg_start, cap + 1, "segments"));
     ICEMPM_TRY(dev_alloc(&ctx->seg_brick, cap, "segment bricks"));
     ICEMPM_TRY(dev_alloc(&ctx->nseg, 1, "segment count"));
-    ICEMPM_TRY(dev_alloc(&ctx->work, cap + 1, "crushed worklist"));
-    ICEMPM_TRY(dev_alloc(&ctx->grid, (size_t)ctx->kp.grid_pitch * 4, "grid"));
-    ICEMPM_TRY(dev_alloc(&ctx->brick_flag, (size_t)nb, "brick flags"));
-    ICEMPM_TRY(dev_alloc(&ctx->brick_count, 2, "brick counts"));
+    ICEMPM_TRY(dev_alloc(&ctx->work, 2 * cap + 2, "G2P worklists"));
     ctx->nbins = (long long)prms->IndenterSubdivisions * prms->GridZ * 3;
     ICEMPM_TRY(dev_alloc(&ctx->bins, (size_t)ctx->nbins, "indenter bins"));
     ICEMPM_TRY(dev_alloc(&ctx->err, 1, "error indicator"));
-    ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(), nb, ctx->stream));
+    for (int i = 0; i < 2; i++) ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(i), nb, ctx->stream));
+    ctx->g = 0;
     ICEMPM_TRY(zero_bins(ctx));
     ICEMPM_TRY(clear_err(ctx));
     k_fill_int<<<1, 1, 0, ctx->stream>>>(ctx->nseg, 1, 0);
     ICEMPM_HIP(hipGetLastError());
     k_fill_int<<<1, 1, 0, ctx->stream>>>(ctx->seg_start, 1, 0);
     ICEMPM_HIP(hipGetLastError());
-    ctx->cur_list = 0;
     ctx->cur_pts = 0;
     ctx->n = 0;
     ctx->allocated = true;
@@ -512,22 +539,22 @@ int icempm_reset_indenter_force_accumulator(icempm_ctx *ctx) {
 
 int icempm_reset_grid(icempm_ctx *ctx) {
     ICEMPM_TRY(check_ready(ctx));
-    return do_reset_grid(ctx);
+    return do_reset_grid(ctx, ctx->g);
 }
 
 int icempm_p2g(icempm_ctx *ctx) {
     ICEMPM_TRY(check_ready(ctx));
-    return do_p2g(ctx);
+    return do_p2g(ctx, ctx->g);
 }
 
 int icempm_update_nodes(icempm_ctx *ctx, double cx, double cy, double cz) {
     ICEMPM_TRY(check_ready(ctx));
-    return do_update(ctx, cx, cy, cz);
+    return do_update(ctx, ctx->g, cx, cy, cz);
 }
 
 int icempm_g2p(icempm_ctx *ctx, int record_pq) {
     ICEMPM_TRY(check_ready(ctx));
-    return do_g2p(ctx, record_pq);
+    return do_g2p(ctx, ctx->g, record_pq);
 }
 
 int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int simulation_step) {
@@ -536,17 +563,40 @@ int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int si
     const int every = ctx->prms.UpdateEveryNthStep;
     if (every <= 0) return fail(ICEMPM_ERR_INVALID_ARG, "UpdateEveryNthStep must be positive");
     double t = *simulation_time;
-    for (int s = 0; s < n_substeps; s++) {
+    double c[3];
+    int s = 0;
+    while (s < n_substeps) {
         if (ctx->sort_period > 0 && ctx->substeps_since_sort >= ctx->sort_period) ICEMPM_TRY(brick_sort(ctx));
+        // substeps until the next regrouping
+        int run = n_substeps - s;
+        if (ctx->sort_period > 0) run = std::min<long long>(run, ctx->sort_period - ctx->substeps_since_sort);
+        if (!ctx->fuse) run = 1;
+        // first substep of the run: reset, P2G, grid update
         t += ctx->prms.InitialTimeStep;
-        double c[3];
-        for (int d = 0; d < 3; d++) c[d] = ctx->ind.center_initial[d] + t * ctx->ind.velocity[d];
-        ICEMPM_TRY(do_reset_grid(ctx));
-        ICEMPM_TRY(do_p2g(ctx));
-        ICEMPM_TRY(do_update(ctx, c[0], c[1], c[2]));
-        ICEMPM_TRY(do_g2p(ctx, ((simulation_step + s) % every) == 0 ? 1 : 0));
+        indenter_center(ctx, t, c);
+        ICEMPM_TRY(do_reset_grid(ctx, ctx->g));
+        ICEMPM_TRY(do_p2g(ctx, ctx->g));
+        ICEMPM_TRY(do_update(ctx, ctx->g, c[0], c[1], c[2]));
+        // then G2P of substep k fused with P2G of substep k+1 on the other grid
+        for (int k = 0; k + 1 < run; k++) {
+            const int nxt = ctx->g ^ 1;
+            ICEMPM_TRY(do_reset_grid(ctx, nxt));
+            ICEMPM_TRY(do_g2p2g(ctx, ctx->g, nxt, ((simulation_step + s + k) % every) == 0 ? 1 : 0));
+            t += ctx->prms.InitialTimeStep;
+            indenter_center(ctx, t, c);
+            ICEMPM_TRY(do_update(ctx, nxt, c[0], c[1], c[2]));
+            ctx->g = nxt;
+        }
+        ICEMPM_TRY(do_g2p(ctx, ctx->g, ((simulation_step + s + run - 1) % every) == 0 ? 1 : 0));
+        s += run;
     }
     *simulation_time = t;
+    return ICEMPM_OK;
+}
+
+int icempm_set_fusion(icempm_ctx *ctx, int on) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    ctx->fuse = on != 0;
     return ICEMPM_OK;
 }
 
@@ -672,7 +722,7 @@ int icempm_download_grid(icempm_ctx *ctx, double *out) {
     if (!out) return fail(ICEMPM_ERR_INVALID_ARG, "null output");
     const long long nn = (long long)ctx->kp.gx * ctx->kp.gy * ctx->kp.gz;
     if (!ctx->grid_xmajor) ICEMPM_TRY(dev_alloc(&ctx->grid_xmajor, (size_t)nn * 4, "grid copy"));
-    ICEMPM_HIP(launch_grid_to_xmajor(ctx->kp, ctx->grid, ctx->grid_xmajor, ctx->stream));
+    ICEMPM_HIP(launch_grid_to_xmajor(ctx->kp, ctx->grids[ctx->g].grid, ctx->grid_xmajor, ctx->stream));
     ICEMPM_HIP(hipMemcpyAsync(out, ctx->grid_xmajor, sizeof(double) * nn * 4, hipMemcpyDeviceToHost, ctx->stream));
     ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
     return ICEMPM_OK;

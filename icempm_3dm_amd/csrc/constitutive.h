@@ -107,6 +107,7 @@ __device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3]
     double a12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
 #pragma unroll
     for (int k = 0; k < 9; k++) V[k] = (k % 4 == 0) ? 1.0 : 0.0;
+#pragma unroll 1
     for (int sweep = 0; sweep < 12; sweep++) {
         double off = a01 * a01 + a02 * a02 + a12 * a12;
         double dia = a00 * a00 + a11 * a11 + a22 * a22;

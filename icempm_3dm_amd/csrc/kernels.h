@@ -64,8 +64,13 @@ struct SegInfo {
 };
 hipError_t launch_p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, GridState gs, int cur,
                       int *err, int n_blocks, hipStream_t stream);
-// G2P plus the return mapping of crushed particles (work: n ints, work_count: 1 int).
-hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, const double *grid, int record_pq, int *err, int *work,
-                      int *work_count, hipStream_t stream);
+// G2P including the return mapping of crushed particles. lists: 1 + n ints
+// of scratch (spill count, spill list).
+hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid, int record_pq, int *err,
+                      int *lists, int n_blocks, hipStream_t stream);
+// G2P of one substep reading grid_a fused with P2G of the next substep into
+// gs_b (kernels_g2p2g.hip). lists: 1 + n ints of scratch.
+hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid_a, GridState gs_b,
+                        int cur_b, int record_pq, int *err, int *lists, int n_blocks, hipStream_t stream);
 
 } // namespace icempm

@@ -1,0 +1,149 @@
+// kernels_g2p2g.hip — G2P of substep n fused with P2G of substep n+1.
+//
+// P2G of the next substep depends only on the point state G2P has just
+// produced, so one pass per point does both. Per 256-point chunk:
+//   G  read the 14 doubles G2P needs, gather node velocities from grid A
+//      (LDS tile), update and store the point (26 doubles, trial F);
+//   R  return mapping of the chunk's crushed points, lanes re-packed;
+//   P  read the new state back (L2-resident), build the P2G record and
+//      scatter into grid B (LDS tile, flushed with atomics).
+// Per point that is 112 B read + 208 B written to HBM instead of 200 + 112
+// + 208 B for the two separate passes. Points outside their segment's tile
+// at the start of the substep go to a spill list (k_g2p2g_spill).
+#include "g2p_point.h"
+
+namespace icempm {
+
+namespace {
+
+__global__ void k_zero_count(int *c) { *c = 0; }
+
+__global__ __launch_bounds__(kP2GThreads) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
+                                                       const double *__restrict__ grid_a, GridState gs_b, int cur_b,
+                                                       int record_pq, int *err, int *spill, int *spill_count) {
+    __shared__ P2GShared sh;
+    __shared__ double sv[3][kTileAlloc];
+    __shared__ CrushList cl;
+    const int t = threadIdx.x;
+    const BrickList bl = brick_list(gs_b, cur_b);
+    const int nseg = *seg.nseg;
+    const double c4h = 4. * kp.h_inv;
+    ChunkCursor c = cursor_first(seg, nseg, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
+    while (c.valid()) {
+        int org[3], bc[3];
+        tile_origin(kp, c.brick, org, bc);
+        if (c.first()) {
+            // the previous segment's gathers and flush are finished (barrier at its end)
+            VelTileRegs vt;
+            vel_tile_fetch(vt, t, kp, org, grid_a);
+            vel_tile_store(sv, vt, t);
+            p2g_zero_tile(sh, t);
+            __syncthreads();
+        }
+        const int i = c.c0 + t;
+        const bool valid = i < c.end;
+        PointState s;
+        s.util = ICEMPM_STATUS_DISABLED;
+        if (valid) load_g2p_inputs(pts, i, s);
+        const ChunkCursor nx = cursor_next(c, seg);
+        bool updated = false, spilled = false, crushed = false;
+        if (!(s.util & ICEMPM_STATUS_DISABLED)) {
+            int base[3];
+            double fx[3];
+            const int code = locate(kp, s.pos, base, fx);
+            if (code) {
+                raise_error(err, code);
+            } else {
+                const int rx = base[0] - org[0], ry = base[1] - org[1], rz = base[2] - org[2];
+                if ((unsigned)rx <= kTile - 3 && (unsigned)ry <= kTile - 3 && (unsigned)rz <= kTile - 3) {
+                    double w[3][3], d[3][3];
+                    stencil_weights(fx, w, d);
+                    gather_tile(sv, tile_index(rx, ry, rz), w, d, c4h, s.vel, s.Bp);
+                    crushed = g2p_finish(kp, pts, i, s, record_pq);
+                    updated = true;
+                } else {
+                    spilled = true;
+                }
+            }
+        }
+        wave_append(spilled, i, spill, spill_count);
+        // return mapping of the chunk's crushed points, from the stored trial F
+        crush_list_build(cl, t, crushed, i);
+        if (t < cl.n) return_map_point(kp, pts, cl.idx[t]);
+        __syncthreads();
+        // P2G record of the new state, read back from the lines just stored (L2)
+        {
+            PointState n;
+            n.util = ICEMPM_STATUS_DISABLED;
+            n.Jp = 1.0;
+            if (updated) {
+                n.util = double_to_bits(pts.at(ICEMPM_IDX_UTILITY, i));
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    n.pos[a] = pts.at(ICEMPM_POSX + a, i);
+                    n.vel[a] = pts.at(ICEMPM_VELX + a, i);
+                }
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    n.Fe[k] = pts.at(ICEMPM_FE00 + k, i);
+                    n.Bp[k] = pts.at(ICEMPM_BP00 + k, i);
+                }
+                n.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
+            }
+            p2g_record(sh, t, kp, updated, n.util, n.pos, n.vel, n.Fe, n.Bp, n.Jp, org, err);
+        }
+        __syncthreads();
+        p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), kp, gs_b, bl);
+        if (c.last()) p2g_flush(sh, t, kp, org, bc, gs_b, bl);
+        c = nx;
+    }
+}
+
+// Spilled points: G2P gathering from HBM, then the reference-form scatter
+// of the new state straight into grid B.
+__global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView pts,
+                                                             const double *__restrict__ grid_a, GridState gs_b,
+                                                             int cur_b, int record_pq, int *err, const int *spill,
+                                                             const int *spill_count) {
+    const int count = *spill_count;
+    const double c4h = 4. * kp.h_inv;
+    const BrickList bl = brick_list(gs_b, cur_b);
+    for (int k = blockIdx.x * kP2GThreads + threadIdx.x; k < count; k += gridDim.x * kP2GThreads) {
+        const int i = spill[k];
+        PointState s;
+        load_g2p_inputs(pts, i, s);
+        int base[3];
+        double fx[3];
+        (void)locate(kp, s.pos, base, fx); // listed points passed the bounds check
+        double w[3][3], d[3][3];
+        stencil_weights(fx, w, d);
+        gather_global(kp, grid_a, base, w, d, c4h, s.vel, s.Bp);
+        if (g2p_finish(kp, pts, i, s, record_pq)) {
+            return_map_point(kp, pts, i);
+#pragma unroll
+            for (int q = 0; q < 9; q++) s.Fe[q] = pts.at(ICEMPM_FE00 + q, i);
+            s.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
+        }
+        const int code = scatter_point_global(kp, gs_b, bl, s);
+        if (code) raise_error(err, code);
+    }
+}
+
+} // namespace
+
+hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid_a, GridState gs_b,
+                        int cur_b, int record_pq, int *err, int *lists, int n_blocks, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    int *spill_count = lists, *spill = lists + 1;
+    k_zero_count<<<1, 1, 0, stream>>>(spill_count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
+    k_g2p2g<<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, err, spill,
+                                                  spill_count);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_g2p2g_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid_a, gs_b, cur_b, record_pq, err, spill, spill_count);
+    return hipGetLastError();
+}
+
+} // namespace icempm

@@ -1,0 +1,328 @@
+// p2g_tile.h — the LDS-tiled particle-to-grid scatter shared by the P2G
+// kernel and the fused G2P+P2G kernel (partition_kernel_p2g,
+// mpm_kernels.cuh:20-100).
+//
+// A workgroup (256 threads) owns one brick segment (the points whose cell
+// lay in one 4x4x4-cell brick at the last sort) and an 8x8x8-node LDS tile:
+// the brick's 6^3-node stencil footprint plus one node of margin for points
+// that drifted since the sort. Per 256-point sub-chunk:
+//   record   one lane per point: stress, APIC affine matrix M = m C - dt V
+//            (4/h^2) tau, weights -> LDS record (22 doubles);
+//   scatter  the sub-chunk is cut into 28 slices, one per 9-lane group
+//            (7 groups per wave); lane (ox,oy) of a group owns the column of
+//            3 stencil nodes (ox,oy,0..2) and sums their mass and momentum
+//            over consecutive points sharing a base node in registers,
+//            flushing to the tile with ds_add_f64 when the base changes;
+//            a point outside the tile is taken by a group alone and its 27
+//            nodes go straight to HBM with FP64 atomics;
+//   flush    non-zero tile nodes go to HBM with one FP64 atomic per value
+//            and mark their grid bricks active.
+// W (m v + M (x_i - x_p)) is evaluated as W (a + Mh o), a = m v - M f h,
+// Mh = M h, o the node offset. Grid dimensions are limited to 1024 nodes
+// per axis (10-bit packing of an absolute base node).
+#pragma once
+
+#include "constitutive.h"
+#include "kernels.h"
+
+namespace icempm {
+
+constexpr int kP2GThreads = 256;
+constexpr int kP2GChunk = 256;
+constexpr int kTile = 8;
+constexpr int kTileNodes = kTile * kTile * kTile;
+// LDS tile strides in doubles: x stride 67 (= 3 mod 32) and y stride 8 put
+// the 9 node columns of a lane group on distinct banks.
+constexpr int kTileSY = kTile;
+constexpr int kTileSX = 67;
+constexpr int kTileAlloc = (kTile - 1) * kTileSX + (kTile - 1) * kTileSY + kTile;
+constexpr int kRec = 22;                  // m | wx[3] wy[3] wz[3] | a[3] | Mh[9]
+constexpr int kRecStride = kP2GChunk + 1; // odd row stride: rows on different banks
+constexpr int kGroups = (kP2GThreads / 64) * 7;
+
+__device__ __forceinline__ int tile_index(int tx, int ty, int tz) { return tx * kTileSX + ty * kTileSY + tz; }
+
+struct P2GShared {
+    double tile[4][kTileAlloc];
+    double rec[kRec][kRecStride];
+    int base[kP2GChunk];   // tile-relative base node (packed 3:3:3), -1 none, -2 outside the tile
+    int absb[kP2GChunk];   // absolute base node of points outside the tile (packed 10:10:10)
+    int over[kP2GChunk];   // sub-chunk indices of points outside the tile
+    int wave_count[kP2GThreads / 64];
+    int nover;
+    unsigned brickmask;
+};
+
+// Appends brick b to the active list once (its flag guards the append).
+struct BrickList {
+    int32_t *flag;
+    int32_t *list;
+    int32_t *count;
+};
+
+// `cur` picks one of the two lists of GridState by select, not by runtime
+// index (indexing the by-value struct would spill it to scratch).
+__device__ __forceinline__ BrickList brick_list(const GridState &gs, int cur) {
+    return BrickList{gs.brick_flag, cur ? gs.brick_list[1] : gs.brick_list[0], gs.brick_count + (cur ? 1 : 0)};
+}
+
+__device__ __forceinline__ void mark_brick(const BrickList &bl, long long b) {
+    if (bl.flag[b] == 0 && atomicCAS(&bl.flag[b], 0, 1) == 0) {
+        int slot = atomicAdd(bl.count, 1);
+        bl.list[slot] = (int32_t)b;
+    }
+}
+
+// Origin (node coordinates) of the tile of brick `brick`, and its brick coordinates.
+__device__ __forceinline__ void tile_origin(const KParams &kp, int brick, int org[3], int bc[3]) {
+    bc[2] = brick % kp.nbz;
+    bc[1] = (brick / kp.nbz) % kp.nby;
+    bc[0] = brick / (kp.nbz * kp.nby);
+    for (int d = 0; d < 3; d++) org[d] = bc[d] * 4 - 2;
+}
+
+__device__ __forceinline__ void p2g_zero_tile(P2GShared &sh, int t) {
+    for (int k = t; k < 4 * kTileAlloc; k += kP2GThreads) (&sh.tile[0][0])[k] = 0.0;
+    if (t == 0) sh.brickmask = 0u;
+}
+
+// Builds the LDS record of point slot t from its state (the body of
+// partition_kernel_p2g up to the scatter, mpm_kernels.cuh:27-81). Points
+// with a stencil outside the grid raise the error code and get no record.
+__device__ __forceinline__ void p2g_record(P2GShared &sh, int t, const KParams &kp, bool valid, long long util,
+                                           const double pos[3], const double vel[3], const double Fe[9],
+                                           const double Bp[9], double Jp_inv, const int org[3], int *err) {
+    int packed = -1;
+    if (valid && !(util & ICEMPM_STATUS_DISABLED)) {
+        const bool liquid = util & ICEMPM_STATUS_LIQUID;
+        const double m = liquid ? kp.mass_liquid : kp.mass_solid;
+        int base[3];
+        double fx[3];
+        const int code = locate(kp, pos, base, fx);
+        if (code) {
+            raise_error(err, code);
+        } else {
+            double sub[9];
+            {
+                double PFt[9];
+                if (liquid) water_stress(kp, Jp_inv, PFt);
+                else kirchhoff_stress(kp, Fe, PFt);
+#pragma unroll
+                for (int k = 0; k < 9; k++) sub[k] = m * Bp[k] - kp.dt_vol_Dpinv * PFt[k];
+            }
+            const double h = kp.h;
+            const double fh[3] = {fx[0] * h, fx[1] * h, fx[2] * h};
+            sh.rec[0][t] = m;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double wd[3];
+                bspline3(fx[d], wd);
+                sh.rec[1 + 3 * d][t] = wd[0];
+                sh.rec[2 + 3 * d][t] = wd[1];
+                sh.rec[3 + 3 * d][t] = wd[2];
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                sh.rec[10 + r][t] = vel[r] * m - (sub[3 * r] * fh[0] + sub[3 * r + 1] * fh[1] + sub[3 * r + 2] * fh[2]);
+#pragma unroll
+                for (int c = 0; c < 3; c++) sh.rec[13 + 3 * r + c][t] = sub[3 * r + c] * h;
+            }
+            const int rx = base[0] - org[0], ry = base[1] - org[1], rz = base[2] - org[2];
+            if ((unsigned)rx <= kTile - 3 && (unsigned)ry <= kTile - 3 && (unsigned)rz <= kTile - 3) {
+                packed = rx | (ry << 3) | (rz << 6);
+            } else {
+                packed = -2;
+                sh.absb[t] = base[0] | (base[1] << 10) | (base[2] << 20);
+            }
+        }
+    }
+    sh.base[t] = packed;
+}
+
+// Column accumulator of one group lane: mass and momentum of the stencil
+// nodes (ox, oy, 0..2) of the current base node.
+struct Column {
+    double m[3], v0[3], v1[3], v2[3];
+};
+
+__device__ __forceinline__ void column_zero(Column &c) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) c.m[k] = c.v0[k] = c.v1[k] = c.v2[k] = 0.0;
+}
+
+// Adds record j: W (m, a + Mh o) for o = (ox, oy, k).
+__device__ __forceinline__ void column_add(const P2GShared &sh, int j, int ox, int oy, double dox, double doy,
+                                           Column &c) {
+    const double wxy = sh.rec[1 + ox][j] * sh.rec[4 + oy][j];
+    const double m = sh.rec[0][j];
+    double base[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        base[r] = sh.rec[10 + r][j] + sh.rec[13 + 3 * r][j] * dox + sh.rec[14 + 3 * r][j] * doy;
+    const double mz[3] = {sh.rec[15][j], sh.rec[18][j], sh.rec[21][j]}; // Mh(r, 2)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double W = wxy * sh.rec[7 + k][j];
+        c.m[k] += W * m;
+        const double u0 = (k == 0) ? base[0] : base[0] + mz[0] * (double)k;
+        const double u1 = (k == 0) ? base[1] : base[1] + mz[1] * (double)k;
+        const double u2 = (k == 0) ? base[2] : base[2] + mz[2] * (double)k;
+        c.v0[k] += W * u0;
+        c.v1[k] += W * u1;
+        c.v2[k] += W * u2;
+    }
+}
+
+__device__ __forceinline__ void column_to_tile(P2GShared &sh, int packed, int ox, int oy, const Column &c) {
+    const int tn = tile_index((packed & 7) + ox, ((packed >> 3) & 7) + oy, (packed >> 6) & 7);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        atomicAdd(&sh.tile[0][tn + k], c.m[k]);
+        atomicAdd(&sh.tile[1][tn + k], c.v0[k]);
+        atomicAdd(&sh.tile[2][tn + k], c.v1[k]);
+        atomicAdd(&sh.tile[3][tn + k], c.v2[k]);
+    }
+}
+
+// Scatter of a sub-chunk whose records are complete (call after a barrier;
+// ends with a barrier). nchunk: number of record slots used.
+__device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, const KParams &kp,
+                                            const GridState &gs, const BrickList &bl) {
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    // list of points outside the tile
+    const bool over = sh.base[t] == -2;
+    const unsigned long long om = __ballot(over);
+    if (lane == 0) sh.wave_count[wave] = __popcll(om);
+    __syncthreads();
+    {
+        int offo = 0;
+        for (int w = 0; w < wave; w++) offo += sh.wave_count[w];
+        if (over) sh.over[offo + __popcll(om & ((1ull << lane) - 1ull))] = t;
+        if (t == 0) {
+            int to = 0;
+            for (int w = 0; w < kP2GThreads / 64; w++) to += sh.wave_count[w];
+            sh.nover = to;
+        }
+    }
+    __syncthreads();
+
+    // lane roles: 7 groups of 9 lanes per wave; lane (ox,oy) owns a node column
+    const int gq = lane % 9;
+    const bool group_lane = lane < 63;
+    const int group = wave * 7 + lane / 9;
+    const int ox = gq / 3, oy = gq % 3;
+    const double dox = ox, doy = oy;
+    {
+        const int per = (nchunk + kGroups - 1) / kGroups;
+        const int j_lo = group_lane ? min(group * per, nchunk) : nchunk;
+        const int j_hi = min(j_lo + per, nchunk);
+        Column col;
+        column_zero(col);
+        int cur_pk = -1;
+        for (int j = j_lo; j < j_hi; j++) {
+            const int pk = sh.base[j];
+            if (pk < 0) continue;
+            if (pk != cur_pk) {
+                if (cur_pk >= 0) {
+                    column_to_tile(sh, cur_pk, ox, oy, col);
+                    column_zero(col);
+                }
+                cur_pk = pk;
+            }
+            column_add(sh, j, ox, oy, dox, doy, col);
+        }
+        if (cur_pk >= 0) column_to_tile(sh, cur_pk, ox, oy, col);
+    }
+    const int nover = sh.nover;
+    for (int r = group_lane ? group : nover; r < nover; r += kGroups) {
+        const int j = sh.over[r];
+        const int ab = sh.absb[j];
+        Column col;
+        column_zero(col);
+        column_add(sh, j, ox, oy, dox, doy, col);
+        const int x = (ab & 1023) + ox, y = ((ab >> 10) & 1023) + oy, z0 = (ab >> 20) & 1023;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const long long idx = node_index(kp, x, y, z0 + k);
+            unsafeAtomicAdd(&gs.grid[idx], col.m[k]);
+            unsafeAtomicAdd(&gs.grid[kp.grid_pitch + idx], col.v0[k]);
+            unsafeAtomicAdd(&gs.grid[2 * kp.grid_pitch + idx], col.v1[k]);
+            unsafeAtomicAdd(&gs.grid[3 * kp.grid_pitch + idx], col.v2[k]);
+        }
+        // the corner columns reach every brick the stencil touches
+        if (ox != 1 && oy != 1) {
+            mark_brick(bl, brick_of(kp, x, y, z0));
+            mark_brick(bl, brick_of(kp, x, y, z0 + 2));
+        }
+    }
+    __syncthreads();
+}
+
+// Adds the tile to the grid and marks the touched bricks (call after a
+// barrier; ends with a barrier).
+__device__ __forceinline__ void p2g_flush(P2GShared &sh, int t, const KParams &kp, const int org[3],
+                                          const int bc[3], const GridState &gs, const BrickList &bl) {
+    for (int k = t; k < kTileNodes; k += kP2GThreads) {
+        const int tx = k >> 6, ty = (k >> 3) & 7, tz = k & 7;
+        const int tk = tile_index(tx, ty, tz);
+        const double m = sh.tile[0][tk];
+        if (m != 0.0) {
+            const long long idx = node_index(kp, org[0] + tx, org[1] + ty, org[2] + tz);
+            unsafeAtomicAdd(&gs.grid[idx], m);
+            unsafeAtomicAdd(&gs.grid[kp.grid_pitch + idx], sh.tile[1][tk]);
+            unsafeAtomicAdd(&gs.grid[2 * kp.grid_pitch + idx], sh.tile[2][tk]);
+            unsafeAtomicAdd(&gs.grid[3 * kp.grid_pitch + idx], sh.tile[3][tk]);
+            // tile node t lies in brick (b - 1) + ((t + 2) >> 2) along each axis
+            const int dbx = (tx + 2) >> 2, dby = (ty + 2) >> 2, dbz = (tz + 2) >> 2;
+            atomicOr(&sh.brickmask, 1u << ((dbx * 3 + dby) * 3 + dbz));
+        }
+    }
+    __syncthreads();
+    if (t < 27 && ((sh.brickmask >> t) & 1u)) {
+        const int dbx = t / 9, dby = (t / 3) % 3, dbz = t % 3;
+        const long long b = ((long long)(bc[0] - 1 + dbx) * kp.nby + (bc[1] - 1 + dby)) * kp.nbz + (bc[2] - 1 + dbz);
+        mark_brick(bl, b);
+    }
+    __syncthreads();
+}
+
+// Work items of a workgroup: the sub-chunks of its contiguous range of
+// segments, in order.
+struct ChunkCursor {
+    long long s, s_hi;
+    int begin, end, c0, brick;
+    __device__ __forceinline__ bool valid() const { return s < s_hi; }
+    __device__ __forceinline__ bool first() const { return c0 == begin; }
+    __device__ __forceinline__ bool last() const { return c0 + kP2GChunk >= end; }
+};
+
+__device__ __forceinline__ void cursor_load_segment(ChunkCursor &c, const SegInfo &seg) {
+    if (c.s < c.s_hi) {
+        c.begin = seg.seg_start[c.s];
+        c.end = seg.seg_start[c.s + 1];
+        c.brick = seg.seg_brick[c.s];
+        c.c0 = c.begin;
+    }
+}
+
+__device__ __forceinline__ ChunkCursor cursor_first(const SegInfo &seg, int nseg, int lb, int nblocks) {
+    ChunkCursor c;
+    c.s = (long long)nseg * lb / nblocks;
+    c.s_hi = (long long)nseg * (lb + 1) / nblocks;
+    c.begin = c.end = c.c0 = c.brick = 0;
+    cursor_load_segment(c, seg);
+    return c;
+}
+
+__device__ __forceinline__ ChunkCursor cursor_next(ChunkCursor c, const SegInfo &seg) {
+    if (c.c0 + kP2GChunk < c.end) {
+        c.c0 += kP2GChunk;
+    } else {
+        c.s++;
+        cursor_load_segment(c, seg);
+    }
+    return c;
+}
+
+} // namespace icempm

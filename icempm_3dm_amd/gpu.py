@@ -75,6 +75,9 @@ class GPUImplementation6:
     def set_sort_period(self, period: int) -> None:
         check(lib().icempm_set_sort_period(self.ctx, int(period)))
 
+    def set_fusion(self, on: bool) -> None:
+        check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))
+
     def transfer_ponts_to_device(self) -> None:
         """GPU_Implementation6::transfer_ponts_to_device (gpu_implementation6.cpp:404-414)."""
         check(lib().icempm_transfer_points_to_device(self.ctx, dptr(self.hssoa.host_buffer), self.hssoa.capacity, 0,

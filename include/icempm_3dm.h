@@ -141,6 +141,12 @@ int icempm_g2p(icempm_ctx *ctx, int record_pq);
  * substep whose global index is a multiple of it. Asynchronous. */
 int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int simulation_step);
 
+/* Extension: with fusion on, icempm_step runs the G2P of each substep and the
+ * P2G of the next in one kernel over two grid buffers; off (the default), it
+ * issues the four phases per substep like the reference. Results agree to
+ * rounding (summation order). */
+int icempm_set_fusion(icempm_ctx *ctx, int on);
+
 /* GPU_Implementation6::synchronize (gpu_implementation6.cpp:513-520) */
 int icempm_synchronize(icempm_ctx *ctx);
 /* Reads the device error indicator (synchronizes). */

@@ -165,8 +165,9 @@ def test_phases_match_oracle(require_gpu, liquid):
 # is therefore held to the oracle's own reproducibility: per field,
 # err(device, oracle serial) <= max(1e-12, 10 * err(oracle slab-parallel,
 # oracle serial)), the two oracle runs differing only in summation order.
-@pytest.mark.parametrize("law,dt", [("sphere", 2e-6), ("cylinder", 2e-6), ("sphere", 1e-5)])
-def test_step_matches_oracle(require_gpu, law, dt):
+@pytest.mark.parametrize("law,dt,fuse", [("sphere", 2e-6, True), ("cylinder", 2e-6, True), ("sphere", 1e-5, True),
+                                         ("sphere", 2e-6, False), ("cylinder", 1e-5, False)])
+def test_step_matches_oracle(require_gpu, law, dt, fuse):
     sc = replace(SMALL, dt=dt)
     if law == "sphere":
         p = scenes.make_params(sc)
@@ -181,6 +182,7 @@ def test_step_matches_oracle(require_gpu, law, dt):
     ref2 = hs.copy()
     g = _gpu_with(p, hs, indenter=(shape, center, vel) if law == "sphere" else None)
     g.set_sort_period(3)
+    g.set_fusion(fuse)
     nsub = 8
     g.reset_indenter_force_accumulator()
     t_dev = g.step(nsub, 0.0, 0)
