@@ -29,6 +29,7 @@ OK, ERR_INVALID_ARG, ERR_HIP, ERR_NOT_READY, ERR_CAPACITY, ERR_DEVICE_FLAG, ERR_
 DEVERR_NONFINITE = 76
 INDENTER_CYLINDER_Z, INDENTER_SPHERE = 0, 1
 PHASES = ("sort", "reset", "p2g", "update", "g2p")
+STATS = ("spilled", "sorts", "segments")
 
 
 class SimParams3D(C.Structure):
@@ -167,6 +168,7 @@ _PROTOS = {
     "icempm_sort_by_brick": (C.c_int, [_ctxp]),
     "icempm_set_sort_period": (C.c_int, [_ctxp, C.c_int]),
     "icempm_set_fusion": (C.c_int, [_ctxp, C.c_int]),
+    "icempm_read_stats": (C.c_int, [_ctxp, _i64p, C.c_int]),
     "icempm_cell_keys": (C.c_int, [_ctxp, C.c_int, _u32p]),
     "icempm_device_order": (C.c_int, [_ctxp, _i32p]),
     "icempm_radix_sort_pairs": (C.c_int, [_ctxp, _u32p, C.c_int, _u32p, _i32p]),

@@ -92,6 +92,7 @@ struct icempm_ctx {
 
     int sort_period = 0;
     long long substeps_since_sort = 0;
+    long long n_sorts = 0;
 
     // timing
     bool timing = false;
@@ -196,6 +197,8 @@ void fill_kparams(icempm_ctx *ctx) {
     k.tan_phi = P.DP_tan_phi;
     k.dp_threshold_p = P.DP_threshold_p;
     k.grain_var = P.GrainVariability;
+    k.rm_je_new = std::sqrt(-2. * (-P.DP_threshold_p) / P.kappa + 1.);
+    k.rm_cbrt_je_new = std::cbrt(k.rm_je_new);
     k.drag = P.DragGlobal;
     k.setup_type = P.SetupType;
     k.ind_subdiv = P.IndenterSubdivisions;
@@ -283,6 +286,7 @@ int brick_sort(icempm_ctx *ctx) {
     ICEMPM_HIP(launch_brick_segments(ctx->keys[0], n, ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->scratch,
                                      ctx->stream));
     ctx->substeps_since_sort = 0;
+    ctx->n_sorts++;
     return tm.finish();
 }
 
@@ -658,6 +662,18 @@ int icempm_sort_by_brick(icempm_ctx *ctx) {
 int icempm_set_sort_period(icempm_ctx *ctx, int period) {
     if (!ctx || period < 0) return fail(ICEMPM_ERR_INVALID_ARG, "bad sort period");
     ctx->sort_period = period;
+    return ICEMPM_OK;
+}
+
+int icempm_read_stats(icempm_ctx *ctx, long long *out, int n_out) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (!out || n_out < 0) return fail(ICEMPM_ERR_INVALID_ARG, "bad stats buffer");
+    int dev[2] = {0, 0};
+    ICEMPM_HIP(hipMemcpyAsync(&dev[0], ctx->work, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipMemcpyAsync(&dev[1], ctx->nseg, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    const long long v[ICEMPM_N_STATS] = {dev[0], ctx->n_sorts, dev[1]};
+    for (int i = 0; i < n_out && i < ICEMPM_N_STATS; i++) out[i] = v[i];
     return ICEMPM_OK;
 }
 

@@ -43,62 +43,73 @@ __device__ __forceinline__ void water_stress(const KParams &kp, double J, double
 }
 
 // ---------------------------------------------------------------- SVD
-// F = U diag(S) V^T with U, V rotations, S[0] >= S[1] >= |S[2]|, S[2] < 0
-// iff det F < 0. V from cyclic Jacobi on F^T F (FP64, converged to machine
-// precision), U and S from a Givens QR of F V. This replaces the reference's
-// FP32 McAdams routine (helper_math.cuh:23-1119) with an FP64 one.
+// Reciprocal and reciprocal square root to about 1 ulp: the hardware
+// estimates (relative error below 2^-22) refined by one third-order step,
+// y (1 + e + e^2) with e = 1 - x y for 1/x, and y (1 + r/2 + 3r^2/8) with
+// r = 1 - x y^2 for 1/sqrt(x). Arguments are finite and positive.
+__device__ __forceinline__ double rcp_f64(double x) {
+    const double y = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, y, 1.0);
+    return fma(y, fma(e, e, e), y);
+}
+
+__device__ __forceinline__ double rsq_f64(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double r = fma(-(x * y), y, 1.0);
+    return fma(y * r, fma(r, 0.375, 0.5), y);
+}
+
+// sqrt(x) for x >= 0 (0 maps to 0; negative or NaN x gives NaN).
+__device__ __forceinline__ double sqrt_f64(double x) { return x == 0.0 ? 0.0 : x * rsq_f64(x); }
+
+// One cyclic-Jacobi rotation of the symmetric A in the (p,q) plane, zeroing
+// apq with Rutishauser's update; ar_p = A(r,p), ar_q = A(r,q) for the third
+// index r. V accumulates the rotation in its columns p and q.
+// t = tan(phi) = sgn(d) 2apq / (|d| + sqrt(d^2 + 4apq^2)), d = aqq - app,
+// the root of t^2 + 2 theta t - 1 = 0 with |phi| <= pi/4.
 __device__ __forceinline__ void jacobi_rotate(double &app, double &aqq, double &apq, double &arp, double &arq,
                                               double V[9], int p, int q) {
-    if (apq == 0.0) return;
-    double tau = (aqq - app) / (2.0 * apq);
-    double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-    double c = 1.0 / sqrt(1.0 + t * t);
-    double s = t * c;
-    app -= t * apq;
-    aqq += t * apq;
+    const double d = aqq - app;
+    const double e2 = apq + apq;
+    const double r2 = fma(d, d, e2 * e2);
+    const double t0 = (d < 0.0 ? -e2 : e2) * rcp_f64(fabs(d) + r2 * rsq_f64(r2));
+    const double t = apq == 0.0 ? 0.0 : t0;
+    const double c = rsq_f64(fma(t, t, 1.0));
+    const double s = t * c;
+    app = fma(-t, apq, app);
+    aqq = fma(t, apq, aqq);
     apq = 0.0;
-    double rp = arp, rq = arq;
+    const double rp = arp, rq = arq;
     arp = c * rp - s * rq;
     arq = s * rp + c * rq;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        double vp = V[3 * r + p], vq = V[3 * r + q];
+        const double vp = V[3 * r + p], vq = V[3 * r + q];
         V[3 * r + p] = c * vp - s * vq;
         V[3 * r + q] = s * vp + c * vq;
     }
 }
 
-__device__ __forceinline__ void swap_cols(double M[9], int a, int b) {
+__device__ __forceinline__ void swap_negate_cols(double V[9], int a, int b) {
+    // exchanging two columns flips det V; negating one restores +1
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        double t = M[3 * r + a];
-        M[3 * r + a] = M[3 * r + b];
-        M[3 * r + b] = t;
+        const double t = V[3 * r + a];
+        V[3 * r + a] = V[3 * r + b];
+        V[3 * r + b] = -t;
     }
 }
 
-__device__ __forceinline__ void givens_rows(double B[9], double U[9], int p, int q, int col) {
-    // rotate rows p,q of B so that B(q,col) = 0; U accumulates the transpose.
-    double a = B[3 * p + col], b = B[3 * q + col];
-    double r = sqrt(a * a + b * b);
-    double c = 1.0, s = 0.0;
-    if (r > 0.0) {
-        c = a / r;
-        s = b / r;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        double bp = B[3 * p + k], bq = B[3 * q + k];
-        B[3 * p + k] = c * bp + s * bq;
-        B[3 * q + k] = -s * bp + c * bq;
-        double up = U[3 * k + p], uq = U[3 * k + q];
-        U[3 * k + p] = c * up + s * uq;
-        U[3 * k + q] = -s * up + c * uq;
-    }
-}
-
-__device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3], double V[9]) {
-    // A = F^T F
+// F = U diag(S) V^T with U, V rotations (columns U[3r+k], V[3r+k]),
+// S[0] >= S[1] >= |S[2]| and S[2] < 0 iff det F < 0 — the convention of the
+// reference's svd3x3 (mpm_kernels.cuh:585-601), in FP64 instead of its FP32
+// McAdams routine (helper_math.cuh:23-1119).
+// V and S^2 from cyclic Jacobi on F^T F (converged to the last bit), sorted
+// descending; U by Gram-Schmidt on the columns of B = F V, its third column
+// the cross product of the first two, so S[2] = U_2 . B_2 carries the sign of
+// det F. Returns S^2 for the first two values as computed from B (S2[2] =
+// S[2]^2).
+__device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3], double S2[3], double V[9]) {
     double a00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
     double a11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
     double a22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
@@ -109,46 +120,72 @@ __device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3]
     for (int k = 0; k < 9; k++) V[k] = (k % 4 == 0) ? 1.0 : 0.0;
 #pragma unroll 1
     for (int sweep = 0; sweep < 12; sweep++) {
-        double off = a01 * a01 + a02 * a02 + a12 * a12;
-        double dia = a00 * a00 + a11 * a11 + a22 * a22;
-        if (!(off > 1e-34 * dia)) break; // ~ (1e-17)^2 relative: converged
+        const double off = a01 * a01 + a02 * a02 + a12 * a12;
+        const double dia = a00 * a00 + a11 * a11 + a22 * a22;
+        if (!(off > 1e-34 * dia)) break; // off-diagonal below 1e-17 relative (or NaN)
         jacobi_rotate(a00, a11, a01, a02, a12, V, 0, 1);
         jacobi_rotate(a00, a22, a02, a01, a12, V, 0, 2);
         jacobi_rotate(a11, a22, a12, a01, a02, V, 1, 2);
     }
-    // sort eigenvalues descending; each swap negates one column to keep det V = +1
     if (a00 < a11) {
-        double t = a00; a00 = a11; a11 = t;
-        swap_cols(V, 0, 1);
-#pragma unroll
-        for (int r = 0; r < 3; r++) V[3 * r + 1] = -V[3 * r + 1];
+        const double t = a00; a00 = a11; a11 = t;
+        swap_negate_cols(V, 0, 1);
     }
     if (a00 < a22) {
-        double t = a00; a00 = a22; a22 = t;
-        swap_cols(V, 0, 2);
-#pragma unroll
-        for (int r = 0; r < 3; r++) V[3 * r + 2] = -V[3 * r + 2];
+        const double t = a00; a00 = a22; a22 = t;
+        swap_negate_cols(V, 0, 2);
     }
     if (a11 < a22) {
-        double t = a11; a11 = a22; a22 = t;
-        swap_cols(V, 1, 2);
-#pragma unroll
-        for (int r = 0; r < 3; r++) V[3 * r + 2] = -V[3 * r + 2];
+        const double t = a11; a11 = a22; a22 = t;
+        swap_negate_cols(V, 1, 2);
     }
-    // B = F V, then QR: B = U R with U a product of rotations
     double B[9];
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
         for (int c = 0; c < 3; c++) B[3 * r + c] = F[3 * r] * V[c] + F[3 * r + 1] * V[3 + c] + F[3 * r + 2] * V[6 + c];
+    // first column
+    const double bb0 = B[0] * B[0] + B[3] * B[3] + B[6] * B[6];
+    double u0[3] = {1.0, 0.0, 0.0};
+    S[0] = 0.0;
+    if (bb0 > 0.0) {
+        const double r0 = rsq_f64(bb0);
+        S[0] = bb0 * r0;
 #pragma unroll
-    for (int k = 0; k < 9; k++) U[k] = (k % 4 == 0) ? 1.0 : 0.0;
-    givens_rows(B, U, 0, 1, 0);
-    givens_rows(B, U, 0, 2, 0);
-    givens_rows(B, U, 1, 2, 1);
-    S[0] = B[0];
-    S[1] = B[4];
-    S[2] = B[8];
+        for (int k = 0; k < 3; k++) u0[k] = B[3 * k] * r0;
+    }
+    // second column, orthogonalised against the first
+    const double pr = u0[0] * B[1] + u0[1] * B[4] + u0[2] * B[7];
+    double w[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w[k] = fma(-pr, u0[k], B[3 * k + 1]);
+    double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    if (!(ww > 1e-30 * bb0)) {
+        // rank <= 1: any unit vector orthogonal to u0 (cross with the axis
+        // on which u0 is smallest)
+        const double ax = fabs(u0[0]), ay = fabs(u0[1]), az = fabs(u0[2]);
+        if (ax <= ay && ax <= az) { w[0] = 0.0; w[1] = u0[2]; w[2] = -u0[1]; }
+        else if (ay <= az) { w[0] = -u0[2]; w[1] = 0.0; w[2] = u0[0]; }
+        else { w[0] = u0[1]; w[1] = -u0[0]; w[2] = 0.0; }
+        ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    }
+    const double r1 = rsq_f64(ww);
+    double u1[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) u1[k] = w[k] * r1;
+    S[1] = u1[0] * B[1] + u1[1] * B[4] + u1[2] * B[7];
+    const double u2[3] = {u0[1] * u1[2] - u0[2] * u1[1], u0[2] * u1[0] - u0[0] * u1[2],
+                          u0[0] * u1[1] - u0[1] * u1[0]};
+    S[2] = u2[0] * B[2] + u2[1] * B[5] + u2[2] * B[8];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        U[3 * k] = u0[k];
+        U[3 * k + 1] = u1[k];
+        U[3 * k + 2] = u2[k];
+    }
+    S2[0] = bb0;
+    S2[1] = S[1] * S[1];
+    S2[2] = S[2] * S[2];
 }
 
 // ---------------------------------------------------------------- plasticity
@@ -157,6 +194,7 @@ struct PState {
     double Fe[9];
     double Jp_inv;
     double p_tr, q_tr, Je_tr;
+    double rc_Jsq; // J^{-2/3} (rcbrt(Je_tr^2)), from compute_pq
     long long util;
 };
 
@@ -179,7 +217,8 @@ __device__ __forceinline__ void compute_pq(const KParams &kp, PState &p) {
     double sq = 0;
 #pragma unroll
     for (int k = 0; k < 9; k++) sq += b[k] * b[k];
-    p.q_tr = kCoeff1 * kp.mu * rcbrt(Jsq) * sqrt(sq);
+    p.rc_Jsq = rcbrt(Jsq);
+    p.q_tr = kCoeff1 * kp.mu * p.rc_Jsq * sqrt(sq);
 }
 
 // GetParametersForGrain + CheckIfPointIsInsideFailureSurface (mpm_kernels.cuh:434-474)
@@ -221,57 +260,49 @@ __device__ __forceinline__ void usvt(const double U[9], const double s[3], const
             out[3 * r + c] = US[3 * r] * V[3 * c] + US[3 * r + 1] * V[3 * c + 1] + US[3 * r + 2] * V[3 * c + 2];
 }
 
-// ComputeSVD + Wolper_Drucker_Prager (mpm_kernels.cuh:486-544), run from
-// a separate kernel: only crushed particles take it.
+// ComputeSVD + Wolper_Drucker_Prager (mpm_kernels.cuh:486-544) on a
+// crushed point after compute_pq. J^{2/3} is taken as J^2 * (J^{-2/3})^2 and
+// the tension-branch constants come precomputed; the rest keeps the
+// reference's expression order.
 __device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
-    double U[9], V[9], sig[3];
-    svd3(p.Fe, U, sig, V);
-    double sig2[3] = {sig[0] * sig[0], sig[1] * sig[1], sig[2] * sig[2]};
-    const double Je_tr_sq = p.Je_tr * p.Je_tr;
-    const double mu = kp.mu;
-    double s = mu * rcbrt(Je_tr_sq);
-    double mean = (sig2[0] + sig2[1] + sig2[2]) / 3;
-    double shat[3] = {s * (sig2[0] - mean), s * (sig2[1] - mean), s * (sig2[2] - mean)};
-
-    const double kappa = kp.kappa;
-    const double tan_phi = kp.tan_phi;
-    const double DP_threshold_p = kp.dp_threshold_p;
-    const double pmax = kp.ice_comp;
-    const double qmax = kp.ice_shear;
-
-    if (p.p_tr < -DP_threshold_p || p.Jp_inv < 1) {
-        double p_new = -DP_threshold_p;
-        double Je_new = sqrt(-2. * p_new / kappa + 1.);
-        double cbrt_Je_new = cbrt(Je_new);
-        double snew[3] = {cbrt_Je_new, cbrt_Je_new, cbrt_Je_new};
-        usvt(U, snew, V, p.Fe);
-        p.Jp_inv *= Je_new / p.Je_tr;
+    double U[9], V[9], sig[3], sig2[3];
+    svd3(p.Fe, U, sig, sig2, V);
+    double snew[3];
+    bool project = true;
+    if (p.p_tr < -kp.dp_threshold_p || p.Jp_inv < 1) {
+        snew[0] = snew[1] = snew[2] = kp.rm_cbrt_je_new;
+        p.Jp_inv *= kp.rm_je_new / p.Je_tr;
     } else {
+        const double pmax = kp.ice_comp;
         double q_n_1;
         if (p.p_tr > pmax) {
             q_n_1 = 0;
         } else {
-            double q_from_dp = (p.p_tr + DP_threshold_p) * tan_phi;
+            const double q_from_dp = (p.p_tr + kp.dp_threshold_p) * kp.tan_phi;
             const double pmin = -kp.ice_tens;
-            double q_from_failure_surface = 2 * sqrt((pmax - p.p_tr) * (p.p_tr - pmin)) * qmax / (pmax - pmin);
+            const double q_from_failure_surface =
+                2 * sqrt((pmax - p.p_tr) * (p.p_tr - pmin)) * kp.ice_shear / (pmax - pmin);
             q_n_1 = fmin(q_from_failure_surface, q_from_dp);
         }
-        if (p.q_tr >= q_n_1) {
-            double s_hat_n_1_norm = q_n_1 * kCoeff1Inv;
-            double n2 = shat[0] * shat[0] + shat[1] * shat[1] + shat[2] * shat[2];
-            double nv[3] = {shat[0], shat[1], shat[2]};
+        project = p.q_tr >= q_n_1;
+        if (project) {
+            const double Je_tr_sq = p.Je_tr * p.Je_tr;
+            const double s = kp.mu * p.rc_Jsq;
+            const double mean = (sig2[0] + sig2[1] + sig2[2]) / 3;
+            double nv[3] = {s * (sig2[0] - mean), s * (sig2[1] - mean), s * (sig2[2] - mean)};
+            // Eigen normalized(): unchanged when the norm is zero
+            const double n2 = nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2];
             if (n2 > 0) {
-                double nn = sqrt(n2);
+                const double rn = rsq_f64(n2);
 #pragma unroll
-                for (int k = 0; k < 3; k++) nv[k] = nv[k] / nn;
+                for (int k = 0; k < 3; k++) nv[k] *= rn;
             }
-            double coef = s_hat_n_1_norm * cbrt(Je_tr_sq) / mu;
-            double snew[3];
+            const double coef = q_n_1 * kCoeff1Inv * (Je_tr_sq * p.rc_Jsq * p.rc_Jsq) / kp.mu;
 #pragma unroll
-            for (int k = 0; k < 3; k++) snew[k] = sqrt(coef * nv[k] + mean);
-            usvt(U, snew, V, p.Fe);
+            for (int k = 0; k < 3; k++) snew[k] = sqrt_f64(coef * nv[k] + mean);
         }
     }
+    if (project) usvt(U, snew, V, p.Fe);
 }
 
 } // namespace icempm

@@ -29,6 +29,9 @@ struct KParams {
     double mu, kappa;
     double ice_comp, ice_tens, ice_shear, ice_tens2, nacc_beta;
     double tan_phi, dp_threshold_p, grain_var;
+    // Tension-branch constants of the return mapping (mpm_kernels.cuh:501-503):
+    // Je_new = sqrt(2 DP_threshold_p / kappa + 1) and its cube root.
+    double rm_je_new, rm_cbrt_je_new;
     double drag;
     int setup_type;
 
@@ -96,6 +99,12 @@ __device__ __forceinline__ double det3(const double m[9]) {
     return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
            m[2] * (m[3] * m[7] - m[4] * m[6]);
 }
+
+// Orders the LDS accesses of a wavefront around this point: the fence keeps
+// the compiler from moving memory operations across it, and the LDS executes
+// one wavefront's operations in issue order, so writes before it are seen by
+// every lane's reads after it (no instruction is needed at wavefront scope).
+__device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
 // Maps a linear block index so that each of the 8 XCDs (which receive
 // blocks round-robin) works on one contiguous range of logical blocks.

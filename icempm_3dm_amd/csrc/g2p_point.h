@@ -57,6 +57,32 @@ __device__ __forceinline__ void vel_tile_store(double (*sv)[kTileAlloc], const V
 }
 static_assert(kTileNodes == 2 * kP2GThreads, "two tile nodes per thread");
 
+// One wavefront loads the whole tile: lane l takes the x-planes' nodes
+// (x, l >> 3, l & 7), all 24 loads in flight before the LDS stores.
+__device__ __forceinline__ void vel_tile_load_wave(double (*sv)[kTileAlloc], int lane, const KParams &kp,
+                                                   const int org[3], const double *grid) {
+    static_assert(kTile * kTile == 64, "one tile x-plane per wavefront row");
+    double v[kTile][3];
+    const int y = org[1] + (lane >> 3), z = org[2] + (lane & 7);
+    const bool yz_in = (unsigned)y < (unsigned)(kp.nby * 4) && (unsigned)z < (unsigned)(kp.nbz * 4);
+#pragma unroll
+    for (int tx = 0; tx < kTile; tx++) {
+        const int x = org[0] + tx;
+        v[tx][0] = v[tx][1] = v[tx][2] = 0.0;
+        // nodes outside the allocated bricks are never weighted (their points raised an error)
+        if (yz_in && (unsigned)x < (unsigned)(kp.nbx * 4)) {
+            const long long idx = node_index(kp, x, y, z);
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[tx][c] = grid[(1 + c) * kp.grid_pitch + idx];
+        }
+    }
+    const int t0 = tile_index(0, lane >> 3, lane & 7);
+#pragma unroll
+    for (int tx = 0; tx < kTile; tx++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) sv[c][t0 + tx * kTileSX] = v[tx][c];
+}
+
 // Velocity of the 27 stencil nodes -> point velocity and APIC matrix C
 // (mpm_kernels.cuh:288-305). `V(a, b, c, nv)` fetches node velocity.
 template <typename Fetch>
@@ -135,12 +161,14 @@ __device__ __forceinline__ void gather_global(const KParams &kp, const double *g
            vel, Bp);
 }
 
-// Everything after the gather (mpm_kernels.cuh:307-351) except the SVD
-// return mapping: reads Fe and Jp_inv, then advection, F update, p/q, crush
-// test; stores the 26 (+2
-// with record_pq) lines with the trial F and leaves the new state in s.
-// Returns true for a crushed solid point, whose return mapping
-// (return_map_point) must follow before its F is used.
+// Everything after the gather (mpm_kernels.cuh:307-351): reads Fe and
+// Jp_inv, then advection, F update, p/q, crush test; stores the 26 (+2 with
+// record_pq) lines and leaves the new state in s. With kInlineReturnMap the
+// return mapping of a crushed point runs here and false is returned;
+// otherwise the trial F is stored and true is returned for a crushed solid
+// point, whose return mapping (return_map_point) must follow before its F
+// is used.
+template <bool kInlineReturnMap>
 __device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i, PointState &s, int record_pq) {
     const double dt = kp.dt;
     const bool liquid = s.util & ICEMPM_STATUS_LIQUID;
@@ -176,6 +204,10 @@ __device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i
         compute_pq(kp, p);
         if (!(p.util & ICEMPM_STATUS_CRUSHED)) check_failure_surface(kp, p);
         crushed = (p.util & ICEMPM_STATUS_CRUSHED) != 0;
+        if (kInlineReturnMap && crushed) {
+            return_map(kp, p);
+            crushed = false;
+        }
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) {

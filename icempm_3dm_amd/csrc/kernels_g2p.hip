@@ -1,10 +1,8 @@
 // kernels_g2p.hip — grid-to-particle transfer and constitutive update
-// (partition_kernel_g2p, mpm_kernels.cuh:237-351), one lane per point, one
-// workgroup per brick segment with the segment's node velocities staged in
-// LDS. The return mapping of the crushed points of a 256-point chunk
-// (mpm_kernels.cuh:486-544) runs as a second phase of the workgroup, lanes
-// re-packed over a workgroup-local list, so the SVD registers do not add to
-// the gather's. Point transfer between partitions (:353-398) does not apply.
+// (partition_kernel_g2p, mpm_kernels.cuh:237-351) with the return mapping of
+// crushed points (mpm_kernels.cuh:486-544), one lane per point, one
+// wavefront per brick segment with the segment's node velocities staged in
+// LDS. Point transfer between partitions (:353-398) does not apply.
 #include "g2p_point.h"
 
 namespace icempm {
@@ -13,57 +11,56 @@ namespace {
 
 __global__ void k_zero_count(int *c) { *c = 0; }
 
-// One workgroup per brick segment; a point that drifted outside its
-// segment's tile since the sort goes to the spill list.
+// Each wavefront owns a contiguous range of brick segments and its own LDS
+// copy of a segment's 8x8x8 node velocities; it walks the segment in 64-point
+// chunks, one lane per point, with no workgroup barrier. A point that
+// drifted outside its segment's tile since the sort goes to the spill list.
+constexpr int kG2PWaves = kP2GThreads / 64;
+
 __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, SegInfo seg,
                                                      const double *__restrict__ grid, int record_pq, int *err,
                                                      int *spill, int *spill_count) {
-    __shared__ double sv[3][kTileAlloc];
-    __shared__ CrushList cl;
-    const int t = threadIdx.x;
-    const int nseg = *seg.nseg;
+    __shared__ double sv_all[kG2PWaves][3][kTileAlloc];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double(*sv)[kTileAlloc] = sv_all[wave];
+    const long long nseg = *seg.nseg;
+    const long long nw = (long long)gridDim.x * kG2PWaves;
+    const long long w = (long long)xcd_remap(blockIdx.x, gridDim.x) * kG2PWaves + wave;
     const double c4h = 4. * kp.h_inv;
-    ChunkCursor c = cursor_first(seg, nseg, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
-    while (c.valid()) {
+    for (long long sg = nseg * w / nw, sg_hi = nseg * (w + 1) / nw; sg < sg_hi; sg++) {
+        const int begin = seg.seg_start[sg], end = seg.seg_start[sg + 1];
         int org[3], bc[3];
-        tile_origin(kp, c.brick, org, bc);
-        if (c.first()) {
-            // the previous segment's gathers are done (barrier in crush_list_build)
-            VelTileRegs vt;
-            vel_tile_fetch(vt, t, kp, org, grid);
-            vel_tile_store(sv, vt, t);
-            __syncthreads();
-        }
-        const int i = c.c0 + t;
-        const bool valid = i < c.end;
-        PointState s;
-        s.util = ICEMPM_STATUS_DISABLED;
-        if (valid) load_g2p_inputs(pts, i, s);
-        const ChunkCursor nx = cursor_next(c, seg);
-        bool spilled = false, crushed = false;
-        if (!(s.util & ICEMPM_STATUS_DISABLED)) {
-            int base[3];
-            double fx[3];
-            const int code = locate(kp, s.pos, base, fx);
-            if (code) {
-                raise_error(err, code);
-            } else {
-                const int rx = base[0] - org[0], ry = base[1] - org[1], rz = base[2] - org[2];
-                if ((unsigned)rx <= kTile - 3 && (unsigned)ry <= kTile - 3 && (unsigned)rz <= kTile - 3) {
-                    double w[3][3], d[3][3];
-                    stencil_weights(fx, w, d);
-                    gather_tile(sv, tile_index(rx, ry, rz), w, d, c4h, s.vel, s.Bp);
-                    crushed = g2p_finish(kp, pts, i, s, record_pq);
+        tile_origin(kp, seg.seg_brick[sg], org, bc);
+        // the previous segment's gathers precede these writes
+        wave_lds_sync();
+        vel_tile_load_wave(sv, lane, kp, org, grid);
+        wave_lds_sync();
+        for (int c0 = begin; c0 < end; c0 += 64) {
+            const int i = c0 + lane;
+            PointState s;
+            s.util = ICEMPM_STATUS_DISABLED;
+            if (i < end) load_g2p_inputs(pts, i, s);
+            bool spilled = false;
+            if (!(s.util & ICEMPM_STATUS_DISABLED)) {
+                int base[3];
+                double fx[3];
+                const int code = locate(kp, s.pos, base, fx);
+                if (code) {
+                    raise_error(err, code);
                 } else {
-                    spilled = true;
+                    const int rx = base[0] - org[0], ry = base[1] - org[1], rz = base[2] - org[2];
+                    if ((unsigned)rx <= kTile - 3 && (unsigned)ry <= kTile - 3 && (unsigned)rz <= kTile - 3) {
+                        double w3[3][3], d[3][3];
+                        stencil_weights(fx, w3, d);
+                        gather_tile(sv, tile_index(rx, ry, rz), w3, d, c4h, s.vel, s.Bp);
+                        g2p_finish<true>(kp, pts, i, s, record_pq);
+                    } else {
+                        spilled = true;
+                    }
                 }
             }
+            wave_append(spilled, i, spill, spill_count);
         }
-        wave_append(spilled, i, spill, spill_count);
-        // return mapping of the chunk's crushed points, from the stored trial F
-        crush_list_build(cl, t, crushed, i);
-        if (t < cl.n) return_map_point(kp, pts, cl.idx[t]);
-        c = nx;
     }
 }
 
@@ -82,7 +79,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p_spill(KParams kp, PtsView p
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
         gather_global(kp, grid, base, w, d, c4h, s.vel, s.Bp);
-        if (g2p_finish(kp, pts, i, s, record_pq)) return_map_point(kp, pts, i);
+        g2p_finish<true>(kp, pts, i, s, record_pq);
     }
 }
 

@@ -1,12 +1,13 @@
 // kernels_g2p2g.hip — G2P of substep n fused with P2G of substep n+1.
 //
 // P2G of the next substep depends only on the point state G2P has just
-// produced, so one pass per point does both. Per 256-point chunk:
+// produced, so one pass per point does both. Per 256-point chunk, one lane
+// per point:
 //   G  read the 14 doubles G2P needs, gather node velocities from grid A
-//      (LDS tile), update and store the point (26 doubles, trial F);
-//   R  return mapping of the chunk's crushed points, lanes re-packed;
-//   P  read the new state back (L2-resident), build the P2G record and
-//      scatter into grid B (LDS tile, flushed with atomics).
+//      (LDS tile), update the point (return mapping included) and store it;
+//   P  build the P2G record of the new state from registers, then the
+//      workgroup scatters the chunk into grid B (LDS tile, flushed with
+//      atomics at the end of the segment).
 // Per point that is 112 B read + 208 B written to HBM instead of 200 + 112
 // + 208 B for the two separate passes. Points outside their segment's tile
 // at the start of the substep go to a spill list (k_g2p2g_spill).
@@ -18,12 +19,14 @@ namespace {
 
 __global__ void k_zero_count(int *c) { *c = 0; }
 
-__global__ __launch_bounds__(kP2GThreads) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
+// LDS (two tiles and the records, about 79 KB) already limits the kernel to
+// two workgroups per CU; the bound keeps the compiler within the matching
+// 256 registers instead of spilling past them into AGPRs.
+__global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
                                                        const double *__restrict__ grid_a, GridState gs_b, int cur_b,
                                                        int record_pq, int *err, int *spill, int *spill_count) {
     __shared__ P2GShared sh;
     __shared__ double sv[3][kTileAlloc];
-    __shared__ CrushList cl;
     const int t = threadIdx.x;
     const BrickList bl = brick_list(gs_b, cur_b);
     const int nseg = *seg.nseg;
@@ -46,7 +49,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g(KParams kp, PtsView pts, 
         s.util = ICEMPM_STATUS_DISABLED;
         if (valid) load_g2p_inputs(pts, i, s);
         const ChunkCursor nx = cursor_next(c, seg);
-        bool updated = false, spilled = false, crushed = false;
+        bool updated = false, spilled = false;
         if (!(s.util & ICEMPM_STATUS_DISABLED)) {
             int base[3];
             double fx[3];
@@ -59,7 +62,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g(KParams kp, PtsView pts, 
                     double w[3][3], d[3][3];
                     stencil_weights(fx, w, d);
                     gather_tile(sv, tile_index(rx, ry, rz), w, d, c4h, s.vel, s.Bp);
-                    crushed = g2p_finish(kp, pts, i, s, record_pq);
+                    g2p_finish<true>(kp, pts, i, s, record_pq);
                     updated = true;
                 } else {
                     spilled = true;
@@ -67,31 +70,8 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g(KParams kp, PtsView pts, 
             }
         }
         wave_append(spilled, i, spill, spill_count);
-        // return mapping of the chunk's crushed points, from the stored trial F
-        crush_list_build(cl, t, crushed, i);
-        if (t < cl.n) return_map_point(kp, pts, cl.idx[t]);
-        __syncthreads();
-        // P2G record of the new state, read back from the lines just stored (L2)
-        {
-            PointState n;
-            n.util = ICEMPM_STATUS_DISABLED;
-            n.Jp = 1.0;
-            if (updated) {
-                n.util = double_to_bits(pts.at(ICEMPM_IDX_UTILITY, i));
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    n.pos[a] = pts.at(ICEMPM_POSX + a, i);
-                    n.vel[a] = pts.at(ICEMPM_VELX + a, i);
-                }
-#pragma unroll
-                for (int k = 0; k < 9; k++) {
-                    n.Fe[k] = pts.at(ICEMPM_FE00 + k, i);
-                    n.Bp[k] = pts.at(ICEMPM_BP00 + k, i);
-                }
-                n.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
-            }
-            p2g_record(sh, t, kp, updated, n.util, n.pos, n.vel, n.Fe, n.Bp, n.Jp, org, err);
-        }
+        // P2G record of the new state, straight from registers
+        p2g_record(sh, t, kp, updated, s.util, s.pos, s.vel, s.Fe, s.Bp, s.Jp, org, err);
         __syncthreads();
         p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), kp, gs_b, bl);
         if (c.last()) p2g_flush(sh, t, kp, org, bc, gs_b, bl);
@@ -118,12 +98,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
         gather_global(kp, grid_a, base, w, d, c4h, s.vel, s.Bp);
-        if (g2p_finish(kp, pts, i, s, record_pq)) {
-            return_map_point(kp, pts, i);
-#pragma unroll
-            for (int q = 0; q < 9; q++) s.Fe[q] = pts.at(ICEMPM_FE00 + q, i);
-            s.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
-        }
+        g2p_finish<true>(kp, pts, i, s, record_pq);
         const int code = scatter_point_global(kp, gs_b, bl, s);
         if (code) raise_error(err, code);
     }

@@ -201,6 +201,12 @@ class GPUImplementation6:
                                             capi.i32ptr(perm)))
         return ks, perm
 
+    def read_stats(self) -> dict:
+        """Step counters (synchronizes): spilled points of the last G2P, sorts done, brick segments."""
+        out = np.zeros(len(capi.STATS), dtype=np.int64)
+        check(lib().icempm_read_stats(self.ctx, out.ctypes.data_as(C.POINTER(C.c_longlong)), len(capi.STATS)))
+        return {name: int(out[i]) for i, name in enumerate(capi.STATS)}
+
     # -- timing -----------------------------------------------------------------
     def enable_timing(self, on: bool = True) -> None:
         check(lib().icempm_enable_timing(self.ctx, 1 if on else 0))

@@ -166,6 +166,17 @@ int icempm_sort_by_brick(icempm_ctx *ctx);
 /* Extension: icempm_step regroups by brick every `period` substeps (0: never). */
 int icempm_set_sort_period(icempm_ctx *ctx, int period);
 
+/* Extension: counters of the step (synchronizes). out[ICEMPM_STAT_SPILLED]:
+ * points the last G2P found outside their brick tile (they take the slow
+ * global-memory path); out[ICEMPM_STAT_SORTS]: brick regroupings done since
+ * allocation; out[ICEMPM_STAT_SEGMENTS]: brick segments of the last one.
+ * Writes min(n_out, ICEMPM_N_STATS) values. */
+#define ICEMPM_STAT_SPILLED 0
+#define ICEMPM_STAT_SORTS 1
+#define ICEMPM_STAT_SEGMENTS 2
+#define ICEMPM_N_STATS 3
+int icempm_read_stats(icempm_ctx *ctx, long long *out, int n_out);
+
 /* Keys of the device points in device order (kind 0: reference key,
  * kind 1: brick key; disabled points get 0xFFFFFFFF), copied to keys_out. */
 int icempm_cell_keys(icempm_ctx *ctx, int kind, uint32_t *keys_out);

@@ -156,6 +156,51 @@ def test_phases_match_oracle(require_gpu, liquid):
     g.close()
 
 
+def test_return_map_distorted_f(require_gpu):
+    """Every point crushed, F far from a rotation: the vmax-clamped grid
+    velocities of this state stretch F by up to ~50% in one step, both branches
+    of the return mapping run, and a few points invert (det F < 0). The
+    device's Jacobi-on-F^T F SVD then meets the oracle's one-sided Jacobi to
+    rounding times the conditioning of F, so besides the normwise bar, the
+    per-point error of Fe is held to 1e-13 * cond(F_trial)^2."""
+    p = scenes.make_params(SMALL)
+    shape, center, vel = SMALL.indenter()
+    hs = scenes.perturbed_points(SMALL, seed=5, amp_F=0.3, crushed_fraction=1.0)
+    ref = hs.copy()
+    g = _gpu_with(p, hs, indenter=(shape, center, vel))
+    g.reset_grid()
+    g.p2g()
+    ind = capi.Indenter.make(shape, center, vel)
+    grid_ref = orc.grid_zeros(p)
+    assert orc.p2g(p, ref.host_buffer, ref.capacity, ref.size, grid_ref) == 0
+    t = p.InitialTimeStep
+    c = [center[d] + t * vel[d] for d in range(3)]
+    g.update_nodes(c)
+    orc.update_nodes(p, ind, c, grid_ref, np.zeros(p.IndenterArraySize()))
+    # the oracle's trial F: G2P from its own grid with the return mapping off
+    # is not available, so cond(F) is bounded from the returned F and the
+    # input F together
+    g.g2p(False)
+    assert orc.g2p(p, ref.host_buffer, ref.capacity, ref.size, grid_ref, False) == 0
+    back = _download(g)
+    n = ref.size
+    errs = compare_points(back.lines(), ref.lines(), n)
+    Fd = back.lines()[capi.FE00:capi.FE00 + 9, :n].T.reshape(-1, 3, 3)
+    Fr = ref.lines()[capi.FE00:capi.FE00 + 9, :n].T.reshape(-1, 3, 3)
+    F0 = hs.lines()[capi.FE00:capi.FE00 + 9, :n].T.reshape(-1, 3, 3)
+    cond = np.maximum(np.linalg.cond(Fr), np.linalg.cond(F0))
+    rel = np.linalg.norm(Fd - Fr, axis=(1, 2)) / np.linalg.norm(Fr, axis=(1, 2))
+    worst = int(np.argmax(rel / cond ** 2))
+    inverted = int(np.count_nonzero(np.linalg.det(F0) < 0))
+    print("errors", errs, "max rel", rel.max(), "worst/cond^2", rel[worst] / cond[worst] ** 2, "cond", cond[worst],
+          "inverted inputs", inverted)
+    assert np.all(rel <= 1e-13 * cond ** 2), (rel[worst], cond[worst])
+    jp = back.lines()[capi.IDX_JP_INV, :n]
+    assert np.count_nonzero(jp != hs.lines()[capi.IDX_JP_INV, :n]) > 0.1 * n, "tension branch not exercised"
+    assert g.device_error_code() == 0
+    g.close()
+
+
 # ------------------------------------------------------------------ several substeps
 # After the first substep, the grid velocity is a sum of stress and affine
 # terms a few hundred times larger than itself, so any change of summation
