@@ -2,15 +2,23 @@
 
 Workload (the 16M-particle throughput configuration): a 200x100x100-cell
 block, 8 points per cell (16,000,000), on the 512x256x256 grid with
-h = 1/64 m, dt = 1e-5 s, E = 9e9 Pa, nu = 0.3, friction angle 60 deg,
-cohesion 2e6 Pa, sphere indenter r = 0.1 m at 1 m/s. One step = one substep
-of the reference's Model3D::Step loop (reset, P2G, grid update, G2P with the
-return mapping), with the brick re-sort every --sort-period steps inside the
-timed region. Points are uploaded before timing (HBM-resident).
+h = 1/64 m, E = 9e9 Pa, nu = 0.3, friction angle 60 deg, cohesion 2e6 Pa,
+sphere indenter r = 0.1 m at 1 m/s. The configuration fixes layout,
+material and indenter but no time step; the headline uses dt = 2e-6 s
+(elastic CFL number 0.47: the P-wave speed sqrt((lambda + 2 mu)/rho) =
+3634 m/s times dt/h). The same workload at dt = 1e-5 s (CFL 2.33: the
+block blows apart, every point crushes and spreads over ~4x more grid
+bricks) is timed after it and reported as "unstable_dt".
+
+One step = one substep of the reference's Model3D::Step loop (reset, P2G,
+grid update, G2P with the return mapping), with the G2P of a step fused with
+the P2G of the next and the brick re-sort every --sort-period steps inside
+the timed region. Points are uploaded before timing (HBM-resident).
 
 Prints one JSON line: particle-updates/s, the HBM roofline of the dominant
-kernel (algorithmic bytes / HIP-event duration on the library's stream) and
-the host-core baseline (the oracle on this host's cores, a bounded sample).
+kernel (algorithmic bytes / HIP-event duration on the library's stream,
+measured traffic from the committed rocprofv3 counter summary) and the
+host-core baseline (the oracle on this host's cores, a bounded sample).
 """
 from __future__ import annotations
 
@@ -35,6 +43,10 @@ HBM_PEAK_GBS = 8000.0
 B_P2G_PT, B_G2P_PT = 200, 320
 B_P2G_NODE, B_UPD_NODE, B_G2P_NODE, B_RESET_NODE = 32, 56, 24, 32
 B_SORT_PT = 24 + 8 + 4 * 16 + 416
+# HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled by the same counters on a copy of known size), keyed by kernel phase
+# and workload; written by hand from the profile summary under profiles/.
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
 
 
 def log(msg: str) -> None:
@@ -74,15 +86,63 @@ def cpu_baseline(scene, prms, hs, max_seconds: float, max_steps: int, threads: i
     }
 
 
+def run_steps(scene, hs, args, dt: float, timed: bool):
+    """Uploads the points, runs warm-up + timed steps at time step dt;
+    returns (elapsed s, phase timings, device error, active nodes)."""
+    from dataclasses import replace
+    from icempm_3dm_amd import GPUImplementation6, scenes
+
+    sc = replace(scene, dt=dt)
+    prms = scenes.make_params(sc)
+    g = GPUImplementation6(0)
+    g.hssoa = hs.copy()
+    g.allocate_arrays(prms, hs.capacity)
+    g.update_constants(prms)
+    shape, center, vel = sc.indenter()
+    g.set_indenter(shape, center, vel)
+    g.set_sort_period(args.sort_period)
+    g.set_fusion(args.fuse != 0)
+    g.transfer_ponts_to_device()
+    t_sim, step = 0.0, 0
+    t_sim = g.step(args.warmup, t_sim, step)
+    step += args.warmup
+    g.synchronize()
+    if timed:
+        g.enable_timing(True)
+        g.reset_timings()
+    g.synchronize()
+    t0 = time.perf_counter()
+    t_sim = g.step(args.steps, t_sim, step)
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    timings = g.read_timings() if timed else {}
+    g.enable_timing(False)
+    err = g.device_error_code()
+    stats = g.read_stats()
+    active_nodes = 0
+    if timed:
+        # nodes holding mass after the last step (for the per-node algorithmic bytes)
+        g.reset_grid()
+        g.p2g()
+        grid = g.download_grid()
+        active_nodes = int(np.count_nonzero(grid[0]))
+        del grid
+    g.close()
+    return elapsed, timings, err, active_nodes, stats
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--scene", default="block_16m")
+    ap.add_argument("--dt", type=float, default=2e-6)
+    ap.add_argument("--unstable-dt", type=float, default=1e-5, help="0: skip the second run")
+    ap.add_argument("--fuse", type=int, default=1)
     ap.add_argument("--sort-period", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-max-steps", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -92,87 +152,68 @@ def main() -> int:
               "single-partition step on one device)", file=sys.stderr)
         return 2
 
-    from icempm_3dm_amd import GPUImplementation6, capi, scenes
+    from dataclasses import replace
+    from icempm_3dm_amd import capi, scenes
 
     if capi.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
-    scene = scenes.SCENES[args.scene]
+    scene = replace(scenes.SCENES[args.scene], dt=args.dt)
     prms = scenes.make_params(scene)
     log(f"bench: generating {scene.name} ({scene.n_points} points)")
     hs = scenes.make_points(scene)
     n = hs.size
 
-    g = GPUImplementation6(0)
-    g.hssoa = hs
-    g.allocate_arrays(prms, hs.capacity)
-    g.update_constants(prms)
-    shape, center, vel = scene.indenter()
-    g.set_indenter(shape, center, vel)
-    g.set_sort_period(args.sort_period)
-    log("bench: uploading")
-    g.transfer_ponts_to_device()
-
-    t_sim, step = 0.0, 0
-    log(f"bench: {args.warmup} warm-up steps")
-    t_sim = g.step(args.warmup, t_sim, step)
-    step += args.warmup
-    g.synchronize()
-    if g.device_error_code() != 0:
-        print(f"bench: device error {g.device_error_code()} during warm-up", file=sys.stderr)
-        return 1
-
-    g.enable_timing(True)
-    g.reset_timings()
-    log(f"bench: {args.steps} timed steps")
-    g.synchronize()
-    t0 = time.perf_counter()
-    t_sim = g.step(args.steps, t_sim, step)
-    g.synchronize()
-    elapsed = time.perf_counter() - t0
-    step += args.steps
-    timings = g.read_timings()
-    g.enable_timing(False)
-    err = g.device_error_code()
-
-    # nodes holding mass after the last step (for the per-node algorithmic bytes)
-    g.reset_grid()
-    g.p2g()
-    grid = g.download_grid()
-    active_nodes = int(np.count_nonzero(grid[0]))
-    del grid
-    g.close()
+    log(f"bench: dt={args.dt}: {args.warmup} warm-up + {args.steps} timed steps")
+    elapsed, timings, err, active_nodes, stats = run_steps(scene, hs, args, args.dt, True)
+    if err != 0:
+        print(f"bench: device error {err}", file=sys.stderr)
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * args.steps / elapsed
-    per_launch_bytes = {
-        "p2g": B_P2G_PT * n + B_P2G_NODE * active_nodes,
-        "update": B_UPD_NODE * active_nodes,
-        "g2p": B_G2P_PT * n + B_G2P_NODE * active_nodes,
-        "reset": B_RESET_NODE * active_nodes,
-        "sort": B_SORT_PT * n,
-    }
+    node_bytes = {"p2g": B_P2G_NODE, "update": B_UPD_NODE, "g2p": B_G2P_NODE, "reset": B_RESET_NODE,
+                  "g2p2g": B_G2P_NODE + B_P2G_NODE, "sort": 0}
+    pt_bytes = {"p2g": B_P2G_PT, "update": 0, "g2p": B_G2P_PT, "reset": 0, "g2p2g": B_G2P_PT + B_P2G_PT,
+                "sort": B_SORT_PT}
+    kernel_of = {"p2g": "k_p2g", "update": "k_update_nodes", "g2p": "k_g2p", "reset": "k_reset_grid",
+                 "g2p2g": "k_g2p2g", "sort": "sort kernels"}
     phases = {}
     for name, (ms, cnt) in timings.items():
         if cnt == 0:
             continue
         avg_ms = ms / cnt
-        phases[name] = {"launches": cnt, "avg_ms": avg_ms, "alg_bytes": per_launch_bytes[name],
-                        "alg_GBs": per_launch_bytes[name] / (avg_ms * 1e-3) / 1e9}
+        b = pt_bytes[name] * n + node_bytes[name] * active_nodes
+        phases[name] = {"kernel": kernel_of[name], "launches": cnt, "avg_ms": avg_ms, "alg_bytes": b,
+                        "alg_GBs": b / (avg_ms * 1e-3) / 1e9}
     dominant = max((k for k in phases if k != "sort"), key=lambda k: phases[k]["avg_ms"] * phases[k]["launches"])
     d = phases[dominant]
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        tr = json.loads(TRAFFIC_FILE.read_text())
+        key = f"{d['kernel']}@{scene.name}@dt={args.dt:g}"
+        if key in tr:
+            traffic = tr[key]["hbm_bytes_per_launch"]
     step_bytes = (B_P2G_PT + B_G2P_PT) * n + (B_P2G_NODE + B_UPD_NODE + B_G2P_NODE + B_RESET_NODE) * active_nodes
     roofline = {
         "bound": "hbm",
-        "kernel": dominant,
+        "kernel": d["kernel"],
         "achieved": round(d["alg_GBs"], 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(d["alg_GBs"] / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "alg_bytes_per_launch": d["alg_bytes"],
         "step_alg_GBs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+
+    unstable = None
+    if args.unstable_dt > 0:
+        log(f"bench: dt={args.unstable_dt}: {args.warmup} warm-up + {args.steps} timed steps")
+        el2, _, err2, _, stats2 = run_steps(scene, hs, args, args.unstable_dt, False)
+        unstable = {"dt": args.unstable_dt, "value": n * args.steps / el2,
+                    "ms_per_step": el2 * 1e3 / args.steps, "device_error": err2,
+                    "brick_segments_at_end": stats2["segments"]}
 
     cpu = None
     if not args.no_cpu:
@@ -194,16 +235,20 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{scene.name}: {n} points ({scene.ppc_axis ** 3}/cell) on a "
-                        f"{scene.grid[0]}x{scene.grid[1]}x{scene.grid[2]} grid, h=1/64, dt={scene.dt}, "
-                        f"sphere indenter, brick re-sort every {args.sort_period} steps",
+                        f"{scene.grid[0]}x{scene.grid[1]}x{scene.grid[2]} grid, h=1/64, dt={args.dt:g}, "
+                        f"sphere indenter, G2P+P2G fusion {'on' if args.fuse else 'off'}, "
+                        f"brick re-sort every {args.sort_period} steps",
             "particles": n,
             "grid": list(scene.grid),
+            "dt": args.dt,
             "active_nodes": active_nodes,
+            "brick_segments_at_end": stats["segments"],
             "device_error": err,
         },
         "roofline": roofline,
         "phases": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in phases.items()},
+        "unstable_dt": unstable,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

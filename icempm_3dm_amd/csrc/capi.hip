@@ -323,7 +323,7 @@ int do_g2p(icempm_ctx *ctx, int gi, int record_pq) {
 
 // G2P of one substep from grid ga fused with P2G of the next into grid gb.
 int do_g2p2g(icempm_ctx *ctx, int ga, int gb, int record_pq) {
-    PhaseTimer tm(ctx, ICEMPM_PHASE_G2P);
+    PhaseTimer tm(ctx, ICEMPM_PHASE_G2P2G);
     SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
     ICEMPM_HIP(launch_g2p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[ga].grid, ctx->gstate(gb),
                             ctx->grids[gb].cur, record_pq, ctx->err, ctx->work, ctx->p2g_blocks, ctx->stream));

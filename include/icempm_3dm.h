@@ -68,7 +68,8 @@ typedef struct icempm_Indenter {
 #define ICEMPM_PHASE_P2G 2
 #define ICEMPM_PHASE_UPDATE 3
 #define ICEMPM_PHASE_G2P 4
-#define ICEMPM_N_PHASES 5
+#define ICEMPM_PHASE_G2P2G 5 /* fused G2P + next P2G (icempm_set_fusion) */
+#define ICEMPM_N_PHASES 6
 
 typedef struct icempm_ctx icempm_ctx;
 
