@@ -178,6 +178,8 @@ _PROTOS = {
     "icempm_read_timings": (C.c_int, [_ctxp, _dp, _i64p]),
     "icempm_scene_block": (C.c_int, [_dp, C.c_uint, _ip, C.c_int, _dp, C.c_double, C.c_double,
                                      C.c_ulonglong, C.c_int, _ip]),
+    "icempm_read_raw_points": (C.c_int, [C.c_char_p, _dp, C.c_uint, _ip, _dp]),
+    "icempm_prepare_raw_points": (C.c_int, [_pp, _dp, C.c_uint, C.c_int]),
     "icempm_params_reset": (None, [_pp]),
     "icempm_params_compute_lame": (None, [_pp]),
     "icempm_params_compute_camclay": (None, [_pp]),

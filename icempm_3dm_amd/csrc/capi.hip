@@ -25,6 +25,15 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+} // namespace
+
+namespace icempm {
+// Error reporting for the host-side translation units of the library.
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+} // namespace icempm
+
+namespace {
+
 #define ICEMPM_HIP(expr)                                                                                      \
     do {                                                                                                     \
         hipError_t _e = (expr);                                                                              \

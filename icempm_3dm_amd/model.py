@@ -20,6 +20,11 @@ class Model3D:
         self.gpu.prms = prms
         self.force_history: list[tuple[float, float, float, float]] = []
 
+    def Reset(self) -> None:
+        """model_3d.cpp:122-130: simulation time and step back to zero."""
+        self.prms.SimulationStep = 0
+        self.prms.SimulationTime = 0.0
+
     def Prepare(self) -> None:
         """model_3d.cpp:132-137."""
         self.gpu.update_constants(self.prms)

@@ -198,6 +198,29 @@ int icempm_reset_timings(icempm_ctx *ctx);
  * number of timed launches. Synchronizes. */
 int icempm_read_timings(icempm_ctx *ctx, double *ms_out, long long *launches_out);
 
+/* SnapshotManager::LoadRawPoints, file half (snapshotmanager.cpp:26-47): reads
+ * an HDF5 raw point cloud (1-D datasets "x", "y", "z" of doubles and
+ * "llGrainIDs" of 64-bit integers with a double attribute "volume"). With
+ * hssoa == NULL only *n_points_out (and *volume_out) are set, so the caller can
+ * size its buffer; otherwise positions go to lines 4-6 and the grain ids, as
+ * stored, to line 0 of points [0, n). ICEMPM_ERR_CAPACITY if n > capacity,
+ * ICEMPM_ERR_INVALID_ARG (with a message) for a missing file, dataset or
+ * attribute, or an HDF5 feature outside the supported subset (version 0/1
+ * superblock, version 1 object headers, symbol-table groups, contiguous or
+ * chunked storage with deflate/shuffle/fletcher32 filters). Host only. */
+int icempm_read_raw_points(const char *path, double *hssoa, unsigned capacity, int *n_points_out,
+                           double *volume_out);
+/* SnapshotManager::LoadRawPoints, geometry half (snapshotmanager.cpp:49-104)
+ * on points [0, n) read by icempm_read_raw_points, prms->Volume set: block
+ * boundaries into xmin..zmax, SetupType 2 liquid marking outside BlockCutout
+ * (host_side_soa.cpp:33-53), the offset that centres the block in x and z with
+ * its bottom at y = 2h (:20-31), a fresh state (Jp_inv = 1, Fe = I, other lines
+ * zero; :88-97), the indenter start, ParticleVolume = Volume / n and
+ * ComputeHelperVariables. The reference's host sort by cell
+ * (RemoveDisabledAndSort) is left to icempm_remove_disabled_and_sort after
+ * the upload. Host only. */
+int icempm_prepare_raw_points(icempm_SimParams3D *prms, double *hssoa, unsigned capacity, int n);
+
 /* Synthetic input (no reference counterpart; replaces the HDF5 point cloud of
  * SnapshotManager::LoadRawPoints for generated scenes): a block of
  * cells[0]*cells[1]*cells[2] lattice cells of edge `cell_size` starting at

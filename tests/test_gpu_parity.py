@@ -311,3 +311,53 @@ def test_block_16m_conservation_and_order(require_gpu):
     got = grid[1:].reshape(3, -1).sum(axis=1)
     assert np.all(np.abs(got - mom) <= 1e-11 * scale), (got, mom)
     g.close()
+
+
+# ------------------------------------------------------------------ initial state from a point cloud
+def test_raw_cloud_prepare_sort_and_steps_match_oracle(require_gpu):
+    """The LoadRawPoints path on an unstructured cloud (uniform random points,
+    not a lattice): host placement (icempm_prepare_raw_points), upload, device
+    RemoveDisabledAndSort, then reference-cylinder substeps against the oracle
+    given the same placed points and its stable sort."""
+    import ctypes as C
+    from icempm_3dm_amd.capi import check, dptr, lib
+    rng = np.random.default_rng(17)
+    n = 30000
+    hs = HostSideSOA(n)
+    hs.size = n
+    L = hs.lines()
+    for a, ext in enumerate((0.4, 0.2, 0.25)):
+        L[capi.POSX + a, :n] = ext * rng.random(n)
+    L[capi.IDX_UTILITY, :n] = rng.integers(0, 1000, n).astype(np.uint64).view(np.float64)
+    p = scenes.make_params(replace(SMALL, dt=2e-6))
+    p.SetupType = 0
+    p.IndDepth = p.IndDiameter / 4
+    p.Volume = 0.4 * 0.2 * 0.25
+    check(lib().icempm_prepare_raw_points(C.byref(p), dptr(hs.host_buffer), hs.capacity, n))
+    ref = hs.copy()
+    m = orc.remove_disabled_and_sort(p, ref.host_buffer, ref.capacity, ref.size)
+    assert m == n
+    g = _gpu_with(p, hs)  # reference cylinder law from update_constants
+    assert g.remove_disabled_and_sort() == n
+    back = _download(g)
+    assert np.array_equal(back.lines()[:, :n], ref.lines()[:, :n]), "device sort differs from the oracle's"
+    nsub = 6
+    g.set_sort_period(4)
+    g.reset_indenter_force_accumulator()
+    t_dev = g.step(nsub, 0.0, 0)
+    ind = orc.reference_indenter(p)
+    ref2 = ref.copy()
+    code, t_ref = orc.step(p, ind, ref.host_buffer, ref.capacity, n, orc.grid_zeros(p),
+                           np.zeros(p.IndenterArraySize()), nsub, 0.0, 0, 1)
+    assert code == 0 and t_dev == t_ref
+    code, _ = orc.step(p, ind, ref2.host_buffer, ref2.capacity, n, orc.grid_zeros(p),
+                       np.zeros(p.IndenterArraySize()), nsub, 0.0, 0, 8)
+    assert code == 0
+    back = _download(g)
+    for name, ls in FIELDS.items():
+        e_dev = normwise(back.lines()[ls, :n], ref.lines()[ls, :n])
+        e_self = normwise(ref2.lines()[ls, :n], ref.lines()[ls, :n])
+        assert e_dev <= max(1e-12, 10 * e_self), (name, e_dev, e_self)
+    assert np.array_equal(back.utility()[:n], ref.utility()[:n])
+    assert g.device_error_code() == 0
+    g.close()
