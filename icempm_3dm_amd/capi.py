@@ -180,6 +180,9 @@ _PROTOS = {
                                      C.c_ulonglong, C.c_int, _ip]),
     "icempm_read_raw_points": (C.c_int, [C.c_char_p, _dp, C.c_uint, _ip, _dp]),
     "icempm_prepare_raw_points": (C.c_int, [_pp, _dp, C.c_uint, C.c_int]),
+    "icempm_save_snapshot": (C.c_int, [C.c_char_p, _pp, _dp, C.c_uint, C.c_uint, _dp, C.c_int]),
+    "icempm_read_snapshot": (C.c_int, [C.c_char_p, _pp, _dp, C.c_uint, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                       _dp, C.c_int]),
     "icempm_params_reset": (None, [_pp]),
     "icempm_params_compute_lame": (None, [_pp]),
     "icempm_params_compute_camclay": (None, [_pp]),

@@ -93,7 +93,7 @@ struct icempm_ctx {
     } grids[2];
     int g = 0; // the grid the per-phase calls use, and the last one updated
     long long nbricks = 0;
-    bool fuse = false;
+    bool fuse = true;
     double *bins = nullptr;
     long long nbins = 0;
     int *err = nullptr;

@@ -1,5 +1,5 @@
 """SnapshotManager mirror (simulation/snapshotmanager.{h,cpp}): the initial
-state from a raw HDF5 point cloud.
+state from a raw HDF5 point cloud, and snapshot save / resume.
 
 ``LoadRawPoints`` follows SnapshotManager::LoadRawPoints
 (snapshotmanager.cpp:20-113): read the cloud, place the block and the
@@ -61,6 +61,54 @@ class SnapshotManager:
         gpu.remove_disabled_and_sort()
         m.Reset()
         m.Prepare()
+        self.previous_frame_exists = False
+
+
+    def SaveSnapshot(self, outputDirectory: str, frame: int, compress: bool = False) -> str:
+        """SnapshotManager::SaveSnapshot (snapshotmanager.cpp:299-339) from the
+        host copy (call transfer_from_device first, as the reference saves
+        hssoa after a cycle). Returns the file path."""
+        os.makedirs(outputDirectory, exist_ok=True)
+        path = os.path.join(outputDirectory, "d%05d.h5" % frame)
+        gpu = self.model.gpu
+        hs = gpu.hssoa
+        bins = np.ascontiguousarray(gpu.indenter_sensor_total, dtype=np.float64)
+        if bins.size != self.model.prms.IndenterArraySize():
+            bins = np.zeros(self.model.prms.IndenterArraySize())
+        check(lib().icempm_save_snapshot(os.fsencode(path), C.byref(self.model.prms), dptr(hs.host_buffer),
+                                         hs.capacity, hs.size, dptr(bins), 1 if compress else 0))
+        return path
+
+    def ReadSnapshot(self, fileName: str, partitions: int = 0) -> None:
+        """SnapshotManager::ReadSnapshot (snapshotmanager.cpp:343-405): params,
+        host buffer and indenter accumulators from the file, then allocate,
+        upload, sort by cell on the device if SimulationStep != 0, Prepare."""
+        if not os.path.exists(fileName):
+            raise capi.IcempmError(capi.ERR_INVALID_ARG, "snapshot file does not exist")
+        m = self.model
+        cap = C.c_uint(0)
+        size = C.c_uint(0)
+        check(lib().icempm_read_snapshot(os.fsencode(fileName), C.byref(m.prms), None, 0, C.byref(cap),
+                                         C.byref(size), None, 0))
+        if partitions > 0:
+            m.prms.nPartitions = partitions
+            m.prms.ComputeHelperVariables()
+        hs = HostSideSOA(cap.value)
+        bins = np.zeros(m.prms.IndenterArraySize())
+        check(lib().icempm_read_snapshot(os.fsencode(fileName), None, dptr(hs.host_buffer), hs.capacity, C.byref(cap),
+                                         C.byref(size), dptr(bins), bins.size))
+        hs.size = size.value
+        gpu = m.gpu
+        gpu.hssoa = hs
+        gpu.prms = m.prms
+        gpu.initialize()
+        gpu.allocate_arrays(m.prms, hs.capacity)
+        gpu.update_constants(m.prms)
+        gpu.transfer_ponts_to_device()
+        if m.prms.SimulationStep != 0:
+            gpu.remove_disabled_and_sort()
+        m.Prepare()
+        gpu.indenter_sensor_total = bins
         self.previous_frame_exists = False
 
 

@@ -142,8 +142,8 @@ int icempm_g2p(icempm_ctx *ctx, int record_pq);
  * substep whose global index is a multiple of it. Asynchronous. */
 int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int simulation_step);
 
-/* Extension: with fusion on, icempm_step runs the G2P of each substep and the
- * P2G of the next in one kernel over two grid buffers; off (the default), it
+/* Extension: with fusion on (the default), icempm_step runs the G2P of each
+ * substep and the P2G of the next in one kernel over two grid buffers; off, it
  * issues the four phases per substep like the reference. Results agree to
  * rounding (summation order). */
 int icempm_set_fusion(icempm_ctx *ctx, int on);
@@ -220,6 +220,28 @@ int icempm_read_raw_points(const char *path, double *hssoa, unsigned capacity, i
  * (RemoveDisabledAndSort) is left to icempm_remove_disabled_and_sort after
  * the upload. Host only. */
 int icempm_prepare_raw_points(icempm_SimParams3D *prms, double *hssoa, unsigned capacity, int n);
+
+/* SnapshotManager::SaveSnapshot (snapshotmanager.cpp:299-339): writes the
+ * HDF5 snapshot file `path` ("d%05d.h5" in the reference) with the dataset
+ * "Indenter" (IndenterSubdivisions*GridZ*3 doubles from indenter_bins) and the
+ * dataset "Points" (the whole host buffer, capacity*28 doubles) carrying the
+ * attributes HSSOA_capacity, HSSOA_size and the SimParams3D values of
+ * SaveParametersAsAttributes (parameters_sim_3d.cpp:178-282). compress != 0
+ * stores "Points" in chunks of min(2^20, size) elements, deflated at level 5,
+ * as the reference does. The file is written to path.tmp and renamed. Host only. */
+int icempm_save_snapshot(const char *path, const icempm_SimParams3D *prms, const double *hssoa, unsigned capacity,
+                         unsigned size, const double *indenter_bins, int compress);
+/* SnapshotManager::ReadSnapshot, file half (snapshotmanager.cpp:343-368, :396-399):
+ * *capacity_out / *size_out from HSSOA_capacity / HSSOA_size; prms (if not
+ * NULL) as SimParams3D::ReadParametersFromAttributes (parameters_sim_3d.cpp:
+ * 285-372) leaves it, except that DP_threshold_p is restored too (the
+ * reference opens that attribute but never reads it). With hssoa == NULL only
+ * these are set; otherwise the points go to a buffer of `capacity` >=
+ * HSSOA_capacity (lines re-pitched, the tail zeroed) and, if indenter_bins is
+ * not NULL, the "Indenter" dataset to it (ICEMPM_ERR_CAPACITY if it holds more
+ * than n_indenter_bins values). Host only. */
+int icempm_read_snapshot(const char *path, icempm_SimParams3D *prms, double *hssoa, unsigned capacity,
+                         unsigned *capacity_out, unsigned *size_out, double *indenter_bins, int n_indenter_bins);
 
 /* Synthetic input (no reference counterpart; replaces the HDF5 point cloud of
  * SnapshotManager::LoadRawPoints for generated scenes): a block of

@@ -361,3 +361,51 @@ def test_raw_cloud_prepare_sort_and_steps_match_oracle(require_gpu):
     assert np.array_equal(back.utility()[:n], ref.utility()[:n])
     assert g.device_error_code() == 0
     g.close()
+
+
+# ------------------------------------------------------------------ snapshot save / resume
+def test_snapshot_resume_matches_continuous_run(require_gpu, tmp_path):
+    """SaveSnapshot after 5 substeps, then 4 more substeps two ways: continuing
+    the first context, and ReadSnapshot into a fresh model (which re-sorts the
+    points on the device, as the reference does for SimulationStep != 0).
+    Same state to rounding (points matched by position), same time and step."""
+    from icempm_3dm_amd import Model3D
+    from icempm_3dm_amd.snapshot import SnapshotManager
+    sc = replace(SMALL, dt=2e-6)
+    p = scenes.reference_cylinder_params(sc)
+    p.DP_threshold_p = 1.7e6  # restored on read (the reference would fall back to its default)
+    hs = scenes.perturbed_points(sc, seed=8, amp_F=5e-4, amp_v=0.02, amp_B=1.0)
+    m1 = Model3D(p)
+    m1.gpu.hssoa = hs
+    m1.gpu.allocate_arrays(p, hs.capacity)
+    m1.Prepare()
+    m1.gpu.transfer_ponts_to_device()
+    m1.gpu.reset_indenter_force_accumulator()
+    t = m1.gpu.step(5, 0.0, 0)
+    p.SimulationTime, p.SimulationStep = t, 5
+    m1.gpu.transfer_from_device()
+    path = SnapshotManager(m1).SaveSnapshot(str(tmp_path), 5, compress=True)
+    t_a = m1.gpu.step(4, t, 5)
+    m1.gpu.transfer_from_device()
+    A = m1.gpu.hssoa.copy()
+
+    m2 = Model3D(scenes.make_params(sc))
+    SnapshotManager(m2).ReadSnapshot(path)
+    q = m2.prms
+    assert (q.SimulationTime, q.SimulationStep) == (t, 5)
+    assert q.DP_threshold_p == 1.7e6
+    t_b = m2.gpu.step(4, q.SimulationTime, q.SimulationStep)
+    assert t_b == t_a
+    m2.gpu.transfer_from_device()
+    B = m2.gpu.hssoa
+    n = A.size
+    assert B.size == n
+    La, Lb = A.lines()[:, :n], B.lines()[:, :n]
+    oa = np.lexsort((La[capi.POSX + 2], La[capi.POSX + 1], La[capi.POSX]))
+    ob = np.lexsort((Lb[capi.POSX + 2], Lb[capi.POSX + 1], Lb[capi.POSX]))
+    La, Lb = La[:, oa], Lb[:, ob]
+    for name, ls in FIELDS.items():
+        assert normwise(Lb[ls], La[ls]) <= 1e-11, name
+    assert np.array_equal(Lb[capi.IDX_UTILITY].view(np.int64), La[capi.IDX_UTILITY].view(np.int64))
+    m1.gpu.close()
+    m2.gpu.close()
