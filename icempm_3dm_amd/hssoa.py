@@ -60,3 +60,6 @@ class HostSideSOA:
 
     def count_active(self) -> int:
         return int(np.count_nonzero((self.utility()[:self.size] & capi.STATUS_DISABLED) == 0))
+
+    def count_disabled(self) -> int:
+        return self.size - self.count_active()

@@ -19,6 +19,7 @@ class Model3D:
         self.gpu = GPUImplementation6(device)
         self.gpu.prms = prms
         self.force_history: list[tuple[float, float, float, float]] = []
+        self.rebalances = 0
 
     def Reset(self) -> None:
         """model_3d.cpp:122-130: simulation time and step back to zero."""
@@ -40,4 +41,13 @@ class Model3D:
         self.gpu.transfer_from_device()
         f = self.gpu.indenter_force
         self.force_history.append((t, float((f ** 2).sum() ** 0.5), float(f[0]), float(f[1])))
+        # rebalance (model_3d.cpp:76-108): too little free space or too many
+        # disabled points -> squeeze and re-sort, here on the device
+        hs = self.gpu.hssoa
+        n = max(hs.size, 1)
+        free_space = (hs.capacity - hs.size) / n
+        disabled = hs.count_disabled() / max(hs.capacity, 1)
+        if free_space < p.RebalanceThresholdFreeSpaceRemaining or disabled > p.RebalanceThresholdDisabledPercentage:
+            self.gpu.remove_disabled_and_sort()
+            self.rebalances += 1
         return p.SimulationTime < p.SimulationEndTime

@@ -40,21 +40,36 @@ _PROTOS = {
     "oracle_step": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, C.c_int]),
 }
 
+ORACLE_LIB_NOFMA = ROOT / "oracle" / "lib" / "libmpm_oracle_nofma.so"
+
 _LIB = None
+_LIB_NOFMA = None
+
+
+def _load(path: Path) -> C.CDLL:
+    if not path.exists():
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+    h = C.CDLL(str(path))
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(h, name)
+        fn.restype = res
+        fn.argtypes = args
+    return h
 
 
 def lib() -> C.CDLL:
     global _LIB
     if _LIB is None:
-        if not ORACLE_LIB.exists():
-            subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
-        h = C.CDLL(str(ORACLE_LIB))
-        for name, (res, args) in _PROTOS.items():
-            fn = getattr(h, name)
-            fn.restype = res
-            fn.argtypes = args
-        _LIB = h
+        _LIB = _load(ORACLE_LIB)
     return _LIB
+
+
+def lib_nofma() -> C.CDLL:
+    """The same oracle built without FMA contraction (rounding-spread probe)."""
+    global _LIB_NOFMA
+    if _LIB_NOFMA is None:
+        _LIB_NOFMA = _load(ORACLE_LIB_NOFMA)
+    return _LIB_NOFMA
 
 
 def _d(a):
@@ -133,10 +148,11 @@ def g2p(p: SimParams3D, soa: np.ndarray, pitch: int, n: int, grid: np.ndarray, r
 
 
 def step(p: SimParams3D, ind: Indenter, soa: np.ndarray, pitch: int, n: int, grid: np.ndarray, bins: np.ndarray,
-         n_substeps: int, sim_time: float, sim_step: int, nthreads: int = 1):
+         n_substeps: int, sim_time: float, sim_step: int, nthreads: int = 1, nofma: bool = False):
     t = C.c_double(sim_time)
-    code = lib().oracle_step(C.byref(p), C.byref(ind), _d(soa), pitch, n, _d(grid), _d(bins), n_substeps,
-                             C.byref(t), sim_step, nthreads)
+    h = lib_nofma() if nofma else lib()
+    code = h.oracle_step(C.byref(p), C.byref(ind), _d(soa), pitch, n, _d(grid), _d(bins), n_substeps,
+                         C.byref(t), sim_step, nthreads)
     return code, t.value
 
 

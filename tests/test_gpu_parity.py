@@ -42,6 +42,29 @@ def _download(g: GPUImplementation6) -> HostSideSOA:
     return out
 
 
+def _oracle_spread(p, ind, start: HostSideSOA, nsub: int, ref: HostSideSOA) -> dict:
+    """How far rounding alone moves the oracle's result after nsub substeps,
+    per field: the same substeps with a different P2G summation order
+    (slab-parallel, 8 threads) and with FMA contraction off, against `ref`
+    (serial, contraction on). The stress of a nearly undeformed F cancels
+    catastrophically (dev(F F^T), J^2 - 1), so the last bits of each compiler's
+    contraction choices grow over substeps; the device's compiler makes its
+    own choices, and is held to 10x this spread (and at least 1e-12)."""
+    n = ref.size
+    out = {}
+    runs = []
+    for kw in ({"nthreads": 8}, {"nofma": True}):
+        r = start.copy()
+        code, _ = orc.step(p, ind, r.host_buffer, r.capacity, n, orc.grid_zeros(p),
+                           np.zeros(p.IndenterArraySize()), nsub, 0.0, 0, **kw)
+        assert code == 0
+        runs.append(r)
+    lines = dict(FIELDS, P=[capi.IDX_P], Q=[capi.IDX_Q])
+    for name, ls in lines.items():
+        out[name] = max(normwise(r.lines()[ls, :n], ref.lines()[ls, :n]) for r in runs)
+    return out
+
+
 # ------------------------------------------------------------------ sort
 @pytest.mark.parametrize("n,kind", [(0, "rand"), (1, "rand"), (4095, "rand"), (4097, "dup"), (1 << 20, "rand"),
                                     (300_000, "equal"), (70_001, "max"), (2_000_003, "dup")])
@@ -202,14 +225,16 @@ def test_return_map_distorted_f(require_gpu):
 
 
 # ------------------------------------------------------------------ several substeps
-# After the first substep, the grid velocity is a sum of stress and affine
-# terms a few hundred times larger than itself, so any change of summation
-# order (device atomics vs the oracle's serial loop) moves vel and Bp by
-# ~1e-13 per substep, and at the configured dt = 1e-5 s (elastic CFL number
-# 2.33) unstable modes amplify it further. Over several substeps the device
-# is therefore held to the oracle's own reproducibility: per field,
-# err(device, oracle serial) <= max(1e-12, 10 * err(oracle slab-parallel,
-# oracle serial)), the two oracle runs differing only in summation order.
+# Over several substeps the device cannot be held to a fixed 1e-12: the
+# Kirchhoff stress of a nearly undeformed F is computed through catastrophic
+# cancellation (dev(F F^T), J^2 - 1), the grid velocity is a sum of stress and
+# affine terms far larger than itself, and so the last-bit differences of
+# summation order (atomics) and of FMA contraction (hipcc vs g++) grow from
+# substep to substep, further at the configured dt = 1e-5 s (CFL 2.33). The
+# device is therefore held to the oracle's own reproducibility (_oracle_spread):
+# per field, err(device, oracle) <= max(1e-12, 10 * spread), the spread being
+# the larger of the oracle against itself with another summation order and
+# against itself built without FMA contraction.
 @pytest.mark.parametrize("law,dt,fuse", [("sphere", 2e-6, True), ("cylinder", 2e-6, True), ("sphere", 1e-5, True),
                                          ("sphere", 2e-6, False), ("cylinder", 1e-5, False)])
 def test_step_matches_oracle(require_gpu, law, dt, fuse):
@@ -223,8 +248,8 @@ def test_step_matches_oracle(require_gpu, law, dt, fuse):
             (p.IndVelocity, 0.0, 0.0)
     ind = capi.Indenter.make(shape, center, vel)
     hs = scenes.perturbed_points(sc, seed=21, amp_F=5e-4, amp_v=0.02, amp_B=1.0)
+    start = hs.copy()
     ref = hs.copy()
-    ref2 = hs.copy()
     g = _gpu_with(p, hs, indenter=(shape, center, vel) if law == "sphere" else None)
     g.set_sort_period(3)
     g.set_fusion(fuse)
@@ -236,18 +261,15 @@ def test_step_matches_oracle(require_gpu, law, dt, fuse):
     code, t_ref = orc.step(p, ind, ref.host_buffer, ref.capacity, ref.size, grid_ref, bins_ref, nsub, 0.0, 0, 1)
     assert code == 0
     assert t_dev == t_ref
-    code, _ = orc.step(p, ind, ref2.host_buffer, ref2.capacity, ref2.size, orc.grid_zeros(p),
-                       np.zeros(p.IndenterArraySize()), nsub, 0.0, 0, 8)
-    assert code == 0
+    spread = _oracle_spread(p, ind, start, nsub, ref)
     back = _download(g)
     n = ref.size
     lines = dict(FIELDS, P=[capi.IDX_P], Q=[capi.IDX_Q])
     report = {}
     for name, ls in lines.items():
         e_dev = normwise(back.lines()[ls, :n], ref.lines()[ls, :n])
-        e_self = normwise(ref2.lines()[ls, :n], ref.lines()[ls, :n])
-        report[name] = (e_dev, e_self)
-        assert e_dev <= max(1e-12, 10 * e_self), (name, e_dev, e_self, report)
+        report[name] = (e_dev, spread[name])
+        assert e_dev <= max(1e-12, 10 * spread[name]), (name, e_dev, spread[name], report)
     assert np.array_equal(back.utility()[:n], ref.utility()[:n])
     print(law, dt, "after", nsub, "substeps (device, oracle-self):", report)
     bins_dev = g.read_indenter_bins()
@@ -346,18 +368,16 @@ def test_raw_cloud_prepare_sort_and_steps_match_oracle(require_gpu):
     g.reset_indenter_force_accumulator()
     t_dev = g.step(nsub, 0.0, 0)
     ind = orc.reference_indenter(p)
-    ref2 = ref.copy()
+    start = ref.copy()
     code, t_ref = orc.step(p, ind, ref.host_buffer, ref.capacity, n, orc.grid_zeros(p),
                            np.zeros(p.IndenterArraySize()), nsub, 0.0, 0, 1)
     assert code == 0 and t_dev == t_ref
-    code, _ = orc.step(p, ind, ref2.host_buffer, ref2.capacity, n, orc.grid_zeros(p),
-                       np.zeros(p.IndenterArraySize()), nsub, 0.0, 0, 8)
-    assert code == 0
+    spread = _oracle_spread(p, ind, start, nsub, ref)
     back = _download(g)
     for name, ls in FIELDS.items():
         e_dev = normwise(back.lines()[ls, :n], ref.lines()[ls, :n])
-        e_self = normwise(ref2.lines()[ls, :n], ref.lines()[ls, :n])
-        assert e_dev <= max(1e-12, 10 * e_self), (name, e_dev, e_self)
+        print(name, e_dev, spread[name])
+        assert e_dev <= max(1e-12, 10 * spread[name]), (name, e_dev, spread[name])
     assert np.array_equal(back.utility()[:n], ref.utility()[:n])
     assert g.device_error_code() == 0
     g.close()
