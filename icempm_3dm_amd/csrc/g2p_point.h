@@ -84,36 +84,54 @@ __device__ __forceinline__ void vel_tile_load_wave(double (*sv)[kTileAlloc], int
 }
 
 // Velocity of the 27 stencil nodes -> point velocity and APIC matrix C
-// (mpm_kernels.cuh:288-305). `V(a, b, c, nv)` fetches node velocity.
+// (mpm_kernels.cuh:288-305): vel = sum W v, C = (4/h) sum W v (o - f)^T with
+// W = wx[a] wy[b] wz[c], summed in separable order (over z, then y, then x),
+// 6 FMAs per node instead of 18. `V(a, b, c, nv)` fetches node velocity.
 template <typename Fetch>
 __device__ __forceinline__ void gather(const double w[3][3], const double d[3][3], double c4h, Fetch V,
                                        double vel[3], double Bp[9]) {
+    double wdz[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) wdz[c] = w[2][c] * d[2][c];
 #pragma unroll
     for (int r = 0; r < 3; r++) vel[r] = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; k++) Bp[k] = 0.0;
 #pragma unroll 1
     for (int a = 0; a < 3; a++) {
-        const double wa = w[0][a];
-        const double da = d[0][a];
+        double Ta[3] = {0.0, 0.0, 0.0}, Ty[3] = {0.0, 0.0, 0.0}, Tz[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int b = 0; b < 3; b++)
+        for (int b = 0; b < 3; b++) {
+            double S0[3] = {0.0, 0.0, 0.0}, S1[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const double weight = wa * w[1][b] * w[2][c];
                 double nv[3];
                 V(a, b, c, nv);
-                const double dp[3] = {da, d[1][b], d[2][c]};
-                const double s = c4h * weight;
 #pragma unroll
                 for (int r = 0; r < 3; r++) {
-                    vel[r] += weight * nv[r];
-                    const double sr = s * nv[r];
-#pragma unroll
-                    for (int q = 0; q < 3; q++) Bp[3 * r + q] += sr * dp[q];
+                    S0[r] += w[2][c] * nv[r];
+                    S1[r] += wdz[c] * nv[r];
                 }
             }
+            const double wy = w[1][b], wyd = w[1][b] * d[1][b];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                Ta[r] += wy * S0[r];
+                Ty[r] += wyd * S0[r];
+                Tz[r] += wy * S1[r];
+            }
+        }
+        const double wx = w[0][a], wxd = w[0][a] * d[0][a];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            vel[r] += wx * Ta[r];
+            Bp[3 * r] += wxd * Ta[r];
+            Bp[3 * r + 1] += wx * Ty[r];
+            Bp[3 * r + 2] += wx * Tz[r];
+        }
     }
+#pragma unroll
+    for (int k = 0; k < 9; k++) Bp[k] *= c4h;
 }
 
 __device__ __forceinline__ void stencil_weights(const double fx[3], double w[3][3], double d[3][3]) {

@@ -32,6 +32,10 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
     const int nseg = *seg.nseg;
     const double c4h = 4. * kp.h_inv;
     ChunkCursor c = cursor_first(seg, nseg, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
+    // utility and position of the next chunk are loaded during the scatter
+    PointState pf;
+    pf.util = ICEMPM_STATUS_DISABLED;
+    if (c.valid() && c.c0 + t < c.end) load_g2p_inputs(pts, c.c0 + t, pf);
     while (c.valid()) {
         int org[3], bc[3];
         tile_origin(kp, c.brick, org, bc);
@@ -44,10 +48,10 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
             __syncthreads();
         }
         const int i = c.c0 + t;
-        const bool valid = i < c.end;
         PointState s;
-        s.util = ICEMPM_STATUS_DISABLED;
-        if (valid) load_g2p_inputs(pts, i, s);
+        s.util = pf.util;
+#pragma unroll
+        for (int a = 0; a < 3; a++) s.pos[a] = pf.pos[a];
         const ChunkCursor nx = cursor_next(c, seg);
         bool updated = false, spilled = false;
         if (!(s.util & ICEMPM_STATUS_DISABLED)) {
@@ -73,6 +77,8 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         // P2G record of the new state, straight from registers
         p2g_record(sh, t, kp, updated, s.util, s.pos, s.vel, s.Fe, s.Bp, s.Jp, org, err);
         __syncthreads();
+        pf.util = ICEMPM_STATUS_DISABLED;
+        if (nx.valid() && nx.c0 + t < nx.end) load_g2p_inputs(pts, nx.c0 + t, pf);
         p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), kp, gs_b, bl);
         if (c.last()) p2g_flush(sh, t, kp, org, bc, gs_b, bl);
         c = nx;

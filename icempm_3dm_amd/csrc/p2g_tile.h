@@ -12,7 +12,9 @@
 //            (7 groups per wave); lane (ox,oy) of a group owns the column of
 //            3 stencil nodes (ox,oy,0..2) and sums their mass and momentum
 //            over consecutive points sharing a base node in registers,
-//            flushing to the tile with ds_add_f64 when the base changes;
+//            flushing to the tile with ds_add_f64 when the base changes
+//            (only the lowest node when the base moves one node up in z,
+//            the other two shift down);
 //            a point outside the tile is taken by a group alone and its 27
 //            nodes go straight to HBM with FP64 atomics;
 //   flush    non-zero tile nodes go to HBM with one FP64 atomic per value
@@ -184,6 +186,31 @@ __device__ __forceinline__ void column_to_tile(P2GShared &sh, int packed, int ox
     }
 }
 
+// Adds the column's lowest node (z offset 0) to the tile.
+__device__ __forceinline__ void column_node_to_tile(P2GShared &sh, int packed, int ox, int oy, const Column &c) {
+    const int tn = tile_index((packed & 7) + ox, ((packed >> 3) & 7) + oy, (packed >> 6) & 7);
+    atomicAdd(&sh.tile[0][tn], c.m[0]);
+    atomicAdd(&sh.tile[1][tn], c.v0[0]);
+    atomicAdd(&sh.tile[2][tn], c.v1[0]);
+    atomicAdd(&sh.tile[3][tn], c.v2[0]);
+}
+
+// Moves the column one node down (for a base one node higher in z).
+__device__ __forceinline__ void column_shift(Column &c) {
+    c.m[0] = c.m[1];
+    c.m[1] = c.m[2];
+    c.m[2] = 0.0;
+    c.v0[0] = c.v0[1];
+    c.v0[1] = c.v0[2];
+    c.v0[2] = 0.0;
+    c.v1[0] = c.v1[1];
+    c.v1[1] = c.v1[2];
+    c.v1[2] = 0.0;
+    c.v2[0] = c.v2[1];
+    c.v2[1] = c.v2[2];
+    c.v2[2] = 0.0;
+}
+
 // Scatter of a sub-chunk whose records are complete (call after a barrier;
 // ends with a barrier). nchunk: number of record slots used.
 __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, const KParams &kp,
@@ -225,8 +252,15 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
             if (pk < 0) continue;
             if (pk != cur_pk) {
                 if (cur_pk >= 0) {
-                    column_to_tile(sh, cur_pk, ox, oy, col);
-                    column_zero(col);
+                    if (pk == cur_pk + (1 << 6)) {
+                        // base moved one node up in z (the common case in
+                        // cell order): the column's lowest node is final
+                        column_node_to_tile(sh, cur_pk, ox, oy, col);
+                        column_shift(col);
+                    } else {
+                        column_to_tile(sh, cur_pk, ox, oy, col);
+                        column_zero(col);
+                    }
                 }
                 cur_pk = pk;
             }
