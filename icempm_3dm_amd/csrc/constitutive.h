@@ -11,27 +11,42 @@ namespace icempm {
 constexpr double kCoeff1 = 1.224744871391589;    // sqrt(3/2), mpm_kernels.cuh:11
 constexpr double kCoeff1Inv = 0.8164965809277260; // sqrt(2/3), mpm_kernels.cuh:12
 
-// KirchhoffStress_Wolper (mpm_kernels.cuh:547-557):
-// tau = mu J^{-2/3} dev(F F^T) + (kappa/2)(J^2 - 1) I
-__device__ __forceinline__ void kirchhoff_stress(const KParams &kp, const double F[9], double P[9]) {
-    double Je = det3(F);
-    double b[9];
+// b = F F^T (row-major, symmetric: the six distinct dot products, the
+// mirrored entries copied, which is bitwise what the nine-product form gives).
+__device__ __forceinline__ void left_cauchy_green(const double F[9], double b[9]) {
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+        for (int c = r; c < 3; c++)
             b[3 * r + c] = F[3 * r] * F[3 * c] + F[3 * r + 1] * F[3 * c + 1] + F[3 * r + 2] * F[3 * c + 2];
+    b[3] = b[1];
+    b[6] = b[2];
+    b[7] = b[5];
+}
+
+// KirchhoffStress_Wolper (mpm_kernels.cuh:547-557):
+// tau = mu J^{-2/3} dev(F F^T) + (kappa/2)(J^2 - 1) I, with J = det F and
+// rc = rcbrt(J^2) given (compute_pq yields both for the same F).
+__device__ __forceinline__ void kirchhoff_stress_j(const KParams &kp, const double F[9], double Je, double rc,
+                                                   double P[9]) {
+    double b[9];
+    left_cauchy_green(F, b);
     double tr3 = (b[0] + b[4] + b[8]) / 3;
     b[0] -= tr3;
     b[4] -= tr3;
     b[8] -= tr3;
-    double s = kp.mu * rcbrt(Je * Je);
+    double s = kp.mu * rc;
     double pterm = kp.kappa * 0.5 * (Je * Je - 1.);
 #pragma unroll
     for (int k = 0; k < 9; k++) P[k] = s * b[k];
     P[0] += pterm;
     P[4] += pterm;
     P[8] += pterm;
+}
+
+__device__ __forceinline__ void kirchhoff_stress(const KParams &kp, const double F[9], double P[9]) {
+    const double Je = det3(F);
+    kirchhoff_stress_j(kp, F, Je, rcbrt(Je * Je), P);
 }
 
 // Water (mpm_kernels.cuh:559-566): kappa (1 - J^-3) I
@@ -205,11 +220,7 @@ __device__ __forceinline__ void compute_pq(const KParams &kp, PState &p) {
     double Jsq = p.Je_tr * p.Je_tr;
     p.p_tr = -(0.5 * kp.kappa) * (Jsq - 1.);
     double b[9];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            b[3 * r + c] = F[3 * r] * F[3 * c] + F[3 * r + 1] * F[3 * c + 1] + F[3 * r + 2] * F[3 * c + 2];
+    left_cauchy_green(F, b);
     double tr3 = (b[0] + b[4] + b[8]) / 3;
     b[0] -= tr3;
     b[4] -= tr3;

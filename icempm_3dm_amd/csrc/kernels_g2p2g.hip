@@ -75,7 +75,8 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         }
         wave_append(spilled, i, spill, spill_count);
         // P2G record of the new state, straight from registers
-        p2g_record(sh, t, kp, updated, s.util, s.pos, s.vel, s.Fe, s.Bp, s.Jp, org, err);
+        const double Je = updated ? det3(s.Fe) : 1.0;
+        p2g_record(sh, t, kp, updated, s.util, s.pos, s.vel, s.Fe, Je, rcbrt(Je * Je), s.Bp, s.Jp, org, err);
         __syncthreads();
         pf.util = ICEMPM_STATUS_DISABLED;
         if (nx.valid() && nx.c0 + t < nx.end) load_g2p_inputs(pts, nx.c0 + t, pf);

@@ -46,7 +46,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(KParams kp, PtsView pts, Se
         int org[3], bc[3];
         tile_origin(kp, c.brick, org, bc);
         if (c.first()) p2g_zero_tile(sh, t);
-        p2g_record(sh, t, kp, c.c0 + t < c.end, in.util, in.pos, in.vel, in.Fe, in.Bp, in.Jp, org, err);
+        const double Je = (in.util & ICEMPM_STATUS_DISABLED) ? 1.0 : det3(in.Fe);
+        p2g_record(sh, t, kp, c.c0 + t < c.end, in.util, in.pos, in.vel, in.Fe, Je, rcbrt(Je * Je), in.Bp, in.Jp,
+                   org, err);
         const ChunkCursor nx = cursor_next(c, seg);
         if (nx.valid()) load_p2g_inputs(pts, nx.c0 + t, nx.c0 + t < nx.end, in);
         __syncthreads();
