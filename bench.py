@@ -10,6 +10,11 @@ material and indenter but no time step; the headline uses dt = 2e-6 s
 block blows apart, every point crushes and spreads over ~4x more grid
 bricks) is timed after it and reported as "unstable_dt".
 
+The batched ensemble (64 independent scenes of a material sweep, counts
+log-uniform in [20k, 500k], each on its own 128x64x64 sub-grid, advanced in
+the same launches) is timed last and reported as "ensemble", with its
+throughput as a fraction of the single-scene headline.
+
 One step = one substep of the reference's Model3D::Step loop (reset, P2G,
 grid update, G2P with the return mapping), with the G2P of a step fused with
 the P2G of the next and the brick re-sort every --sort-period steps inside
@@ -131,6 +136,32 @@ def run_steps(scene, hs, args, dt: float, timed: bool):
     return elapsed, timings, err, active_nodes, stats
 
 
+def run_ensemble(args) -> dict:
+    """Warm-up + timed steps of the sweep ensemble; particle-updates/s."""
+    from icempm_3dm_amd.ensemble import BatchedEnsemble, sweep_scenes
+
+    sw = sweep_scenes(args.ensemble, seed=2024, dt=args.dt)
+    ens = BatchedEnsemble(sw)
+    ens.set_fusion(args.fuse != 0)
+    n = ens.n_points
+    ens.step(args.warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    ens.synchronize()
+    elapsed = time.perf_counter() - t0
+    code = 0
+    import ctypes as C
+    from icempm_3dm_amd import capi
+    c = C.c_int(0)
+    capi.check(capi.lib().icempm_error_code(ens.ctx, C.byref(c)))
+    code = c.value
+    counts = [s.n_points for s in sw]
+    ens.close()
+    return {"scenes": len(sw), "points": n, "points_min": min(counts), "points_max": max(counts),
+            "value": n * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps, "device_error": code}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +176,8 @@ def main() -> int:
     ap.add_argument("--cpu-max-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=64, help="scenes of the ensemble run; 0: skip it")
+    ap.add_argument("--skip-headline", action="store_true", help="only the ensemble run (no JSON line)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -158,6 +191,10 @@ def main() -> int:
     if capi.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    if args.skip_headline:
+        log(f"bench: ensemble of {args.ensemble} scenes: {args.warmup} warm-up + {args.steps} timed steps")
+        print(json.dumps({"ensemble": run_ensemble(args)}), flush=True)
+        return 0
     scene = replace(scenes.SCENES[args.scene], dt=args.dt)
     prms = scenes.make_params(scene)
     log(f"bench: generating {scene.name} ({scene.n_points} points)")
@@ -215,6 +252,12 @@ def main() -> int:
                     "ms_per_step": el2 * 1e3 / args.steps, "device_error": err2,
                     "brick_segments_at_end": stats2["segments"]}
 
+    ensemble = None
+    if args.ensemble > 0:
+        log(f"bench: ensemble of {args.ensemble} scenes: {args.warmup} warm-up + {args.steps} timed steps")
+        ensemble = run_ensemble(args)
+        ensemble["fraction_of_single_scene"] = ensemble["value"] / value
+
     cpu = None
     if not args.no_cpu:
         log(f"bench: host-core baseline ({args.cpu_threads} threads, <= {args.cpu_seconds} s)")
@@ -249,6 +292,7 @@ def main() -> int:
         "phases": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in phases.items()},
         "unstable_dt": unstable,
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -176,6 +176,10 @@ _PROTOS = {
     "icempm_enable_timing": (C.c_int, [_ctxp, C.c_int]),
     "icempm_reset_timings": (C.c_int, [_ctxp]),
     "icempm_read_timings": (C.c_int, [_ctxp, _dp, _i64p]),
+    "icempm_allocate_ensemble": (C.c_int, [_ctxp, _pp, C.c_int, C.c_int]),
+    "icempm_ensemble_set_indenter": (C.c_int, [_ctxp, C.c_int, C.POINTER(Indenter)]),
+    "icempm_ensemble_transfer_points": (C.c_int, [_ctxp, _dp, C.c_uint, _ip]),
+    "icempm_ensemble_layout": (C.c_int, [_ctxp, C.c_int, _ip, C.POINTER(C.c_longlong)]),
     "icempm_scene_block": (C.c_int, [_dp, C.c_uint, _ip, C.c_int, _dp, C.c_double, C.c_double,
                                      C.c_ulonglong, C.c_int, _ip]),
     "icempm_read_raw_points": (C.c_int, [C.c_char_p, _dp, C.c_uint, _ip, _dp]),

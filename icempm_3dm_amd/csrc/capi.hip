@@ -103,6 +103,18 @@ struct icempm_ctx {
     long long substeps_since_sort = 0;
     long long n_sorts = 0;
 
+    // ensemble of independent scenes in one stored grid (ens.n == 0: one scene)
+    struct Ensemble {
+        int n = 0;
+        int slot_b[3] = {0, 0, 0}, nslot[3] = {0, 0, 0};
+        std::vector<icempm_SimParams3D> prms;
+        std::vector<icempm_Indenter> ind; // scene-local centers
+        std::vector<int> origin;          // 3 per scene, nodes of the stored grid
+        std::vector<long long> bins_off;
+        std::vector<int> first; // n + 1: each scene's points in upload order
+        KParams *d_slots = nullptr;
+    } ens;
+
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -177,15 +189,33 @@ struct PhaseTimer {
     }
 };
 
-void fill_kparams(icempm_ctx *ctx) {
-    const icempm_SimParams3D &P = ctx->prms;
-    KParams &k = ctx->kp;
+// The reference's indenter law for a parameter set (model_3d.cpp:38,
+// mpm_kernels.cuh:154): z-cylinder from (indenter_x_initial,
+// indenter_y_initial), moving along +x at IndVelocity.
+icempm_Indenter reference_indenter(const icempm_SimParams3D &P) {
+    icempm_Indenter ind{};
+    ind.shape = ICEMPM_INDENTER_CYLINDER_Z;
+    ind.center_initial[0] = P.indenter_x_initial;
+    ind.center_initial[1] = P.indenter_y_initial;
+    ind.center_initial[2] = 0.0;
+    ind.velocity[0] = P.IndVelocity;
+    return ind;
+}
+
+// Kernel parameters of one scene whose node (0,0,0) sits at node `origin` of a
+// stored grid of nb bricks per axis.
+KParams scene_kparams(const icempm_SimParams3D &P, const icempm_Indenter &ind, const int origin[3],
+                      const int nb[3]) {
+    KParams k{};
     k.gx = P.GridXTotal;
     k.gy = P.GridY;
     k.gz = P.GridZ;
-    k.nbx = (P.GridXTotal + 3) / 4;
-    k.nby = (P.GridY + 3) / 4;
-    k.nbz = (P.GridZ + 3) / 4;
+    k.ox = origin[0];
+    k.oy = origin[1];
+    k.oz = origin[2];
+    k.nbx = nb[0];
+    k.nby = nb[1];
+    k.nbz = nb[2];
     k.grid_pitch = (long long)k.nbx * k.nby * k.nbz * kBrickNodes;
     k.h = P.cellsize;
     k.h_inv = P.cellsize_inv;
@@ -212,18 +242,30 @@ void fill_kparams(icempm_ctx *ctx) {
     k.setup_type = P.SetupType;
     k.ind_subdiv = P.IndenterSubdivisions;
     k.ind_rsq = P.IndRSq;
-    if (!ctx->ind_custom) {
-        // the reference law (model_3d.cpp:38, mpm_kernels.cuh:154)
-        ctx->ind.shape = ICEMPM_INDENTER_CYLINDER_Z;
-        ctx->ind.center_initial[0] = P.indenter_x_initial;
-        ctx->ind.center_initial[1] = P.indenter_y_initial;
-        ctx->ind.center_initial[2] = 0.0;
-        ctx->ind.velocity[0] = P.IndVelocity;
-        ctx->ind.velocity[1] = 0.0;
-        ctx->ind.velocity[2] = 0.0;
+    k.ind_shape = ind.shape;
+    for (int d = 0; d < 3; d++) {
+        k.ind_vel[d] = ind.velocity[d];
+        k.ind_c0[d] = ind.center_initial[d] + origin[d] * P.cellsize;
     }
-    k.ind_shape = ctx->ind.shape;
-    for (int d = 0; d < 3; d++) k.ind_vel[d] = ctx->ind.velocity[d];
+    return k;
+}
+
+void fill_kparams(icempm_ctx *ctx) {
+    const icempm_SimParams3D &P = ctx->prms;
+    if (!ctx->ind_custom) ctx->ind = reference_indenter(P);
+    const int origin[3] = {0, 0, 0};
+    int nb[3] = {(P.GridXTotal + 3) / 4, (P.GridY + 3) / 4, (P.GridZ + 3) / 4};
+    if (ctx->ens.n > 0)
+        for (int d = 0; d < 3; d++) nb[d] = ctx->ens.slot_b[d] * ctx->ens.nslot[d];
+    ctx->kp = scene_kparams(P, ctx->ind, origin, nb);
+    if (ctx->ens.n > 0) {
+        ctx->kp.ens_n = ctx->ens.n;
+        for (int d = 0; d < 3; d++) {
+            ctx->kp.ens_slot_b[d] = ctx->ens.slot_b[d];
+            ctx->kp.ens_nslot[d] = ctx->ens.nslot[d];
+        }
+        ctx->kp.ens_slots = ctx->ens.d_slots;
+    }
 }
 
 template <typename T> int dev_alloc(T **p, size_t count, const char *what) {
@@ -257,6 +299,8 @@ void free_all(icempm_ctx *ctx) {
         f(gb.count);
         gb = icempm_ctx::GridBuf{};
     }
+    f(ctx->ens.d_slots);
+    ctx->ens.d_slots = nullptr;
     f(ctx->scratch);
     f(ctx->seg_start);
     f(ctx->seg_brick);
@@ -315,9 +359,10 @@ int do_p2g(icempm_ctx *ctx, int gi) {
     return tm.finish();
 }
 
-int do_update(icempm_ctx *ctx, int gi, double cx, double cy, double cz) {
+int do_update(icempm_ctx *ctx, int gi, double cx, double cy, double cz, double t) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_UPDATE);
-    ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(gi), ctx->grids[gi].cur, cx, cy, cz, ctx->bins, ctx->stream));
+    ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(gi), ctx->grids[gi].cur, cx, cy, cz, t, ctx->bins,
+                                   ctx->stream));
     return tm.finish();
 }
 
@@ -421,18 +466,21 @@ int icempm_destroy(icempm_ctx *ctx) {
     return ICEMPM_OK;
 }
 
-int icempm_allocate(icempm_ctx *ctx, const icempm_SimParams3D *prms, int n_points_capacity) {
-    if (!ctx || !prms) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
-    if (n_points_capacity < 0) return fail(ICEMPM_ERR_INVALID_ARG, "negative capacity");
+namespace {
+
+int check_grid(const icempm_SimParams3D *prms) {
     if (prms->GridXTotal < 3 || prms->GridY < 3 || prms->GridZ < 3)
         return fail(ICEMPM_ERR_INVALID_ARG, "grid must be at least 3 nodes per axis");
     if (prms->IndenterSubdivisions < 0) return fail(ICEMPM_ERR_INVALID_ARG, "negative IndenterSubdivisions");
     if (prms->GridXTotal > 1024 || prms->GridY > 1024 || prms->GridZ > 1024)
         return fail(ICEMPM_ERR_INVALID_ARG, "grid is limited to 1024 nodes per axis");
-    ICEMPM_TRY(set_device(ctx));
-    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
-    free_all(ctx);
-    ctx->prms = *prms;
+    return ICEMPM_OK;
+}
+
+// Device storage for `prms` (the stored grid) and n_points_capacity points;
+// ctx->ens must already describe the scenes of an ensemble (or be empty).
+int alloc_common(icempm_ctx *ctx, const icempm_SimParams3D &prms, int n_points_capacity, long long nbins) {
+    ctx->prms = prms;
     fill_kparams(ctx);
     const long long nb = (long long)ctx->kp.nbx * ctx->kp.nby * ctx->kp.nbz;
     if (nb * kBrickNodes > (1ll << 40) || nb > 0x7FFFFFF)
@@ -460,7 +508,7 @@ int icempm_allocate(icempm_ctx *ctx, const icempm_SimParams3D *prms, int n_point
     ICEMPM_TRY(dev_alloc(&ctx->seg_brick, cap, "segment bricks"));
     ICEMPM_TRY(dev_alloc(&ctx->nseg, 1, "segment count"));
     ICEMPM_TRY(dev_alloc(&ctx->work, 2 * cap + 2, "G2P worklists"));
-    ctx->nbins = (long long)prms->IndenterSubdivisions * prms->GridZ * 3;
+    ctx->nbins = nbins;
     ICEMPM_TRY(dev_alloc(&ctx->bins, (size_t)ctx->nbins, "indenter bins"));
     ICEMPM_TRY(dev_alloc(&ctx->err, 1, "error indicator"));
     for (int i = 0; i < 2; i++) ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(i), nb, ctx->stream));
@@ -478,9 +526,127 @@ int icempm_allocate(icempm_ctx *ctx, const icempm_SimParams3D *prms, int n_point
     return ICEMPM_OK;
 }
 
+} // namespace
+
+int icempm_allocate(icempm_ctx *ctx, const icempm_SimParams3D *prms, int n_points_capacity) {
+    if (!ctx || !prms) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (n_points_capacity < 0) return fail(ICEMPM_ERR_INVALID_ARG, "negative capacity");
+    ICEMPM_TRY(check_grid(prms));
+    ICEMPM_TRY(set_device(ctx));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    free_all(ctx);
+    ctx->ens = icempm_ctx::Ensemble{};
+    return alloc_common(ctx, *prms, n_points_capacity, (long long)prms->IndenterSubdivisions * prms->GridZ * 3);
+}
+
+namespace {
+
+// Per-slot parameter blocks of an ensemble, to the device (empty slots get
+// scene 0's block: no point or active brick ever lies there).
+int ens_upload_slots(icempm_ctx *ctx) {
+    auto &E = ctx->ens;
+    const int nslots = E.nslot[0] * E.nslot[1] * E.nslot[2];
+    const int nb[3] = {E.slot_b[0] * E.nslot[0], E.slot_b[1] * E.nslot[1], E.slot_b[2] * E.nslot[2]};
+    std::vector<KParams> table((size_t)nslots);
+    for (int k = 0; k < nslots; k++) {
+        const int sc = k < E.n ? k : 0;
+        KParams e = scene_kparams(E.prms[sc], E.ind[sc], &E.origin[3 * sc], nb);
+        e.bins_off = E.bins_off[sc];
+        e.ens_n = E.n;
+        for (int d = 0; d < 3; d++) {
+            e.ens_slot_b[d] = E.slot_b[d];
+            e.ens_nslot[d] = E.nslot[d];
+        }
+        e.ens_slots = E.d_slots;
+        table[k] = e;
+    }
+    ICEMPM_HIP(hipMemcpyAsync(E.d_slots, table.data(), sizeof(KParams) * nslots, hipMemcpyHostToDevice,
+                              ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
+}
+
+} // namespace
+
+int icempm_allocate_ensemble(icempm_ctx *ctx, const icempm_SimParams3D *scenes, int n_scenes, int n_points_capacity) {
+    if (!ctx || !scenes) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (n_scenes < 1 || n_scenes > 65536) return fail(ICEMPM_ERR_INVALID_ARG, "n_scenes must be in [1, 65536]");
+    if (n_points_capacity < 0) return fail(ICEMPM_ERR_INVALID_ARG, "negative capacity");
+    int maxg[3] = {0, 0, 0};
+    for (int i = 0; i < n_scenes; i++) {
+        const icempm_SimParams3D &P = scenes[i];
+        ICEMPM_TRY(check_grid(&P));
+        if (P.InitialTimeStep != scenes[0].InitialTimeStep || P.cellsize != scenes[0].cellsize)
+            return fail(ICEMPM_ERR_INVALID_ARG, "ensemble scenes must share InitialTimeStep and cellsize");
+        maxg[0] = std::max(maxg[0], P.GridXTotal);
+        maxg[1] = std::max(maxg[1], P.GridY);
+        maxg[2] = std::max(maxg[2], P.GridZ);
+    }
+    ICEMPM_TRY(set_device(ctx));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    free_all(ctx);
+    icempm_ctx::Ensemble E;
+    E.n = n_scenes;
+    int maxslots[3];
+    for (int d = 0; d < 3; d++) {
+        // at least one empty brick between scene boxes
+        E.slot_b[d] = (maxg[d] + 4 + 3) / 4;
+        maxslots[d] = 1024 / (E.slot_b[d] * 4);
+        if (maxslots[d] < 1) return fail(ICEMPM_ERR_INVALID_ARG, "scene grid too large for an ensemble slot");
+    }
+    E.nslot[2] = std::min(n_scenes, maxslots[2]);
+    E.nslot[1] = std::min((n_scenes + E.nslot[2] - 1) / E.nslot[2], maxslots[1]);
+    E.nslot[0] = (n_scenes + E.nslot[1] * E.nslot[2] - 1) / (E.nslot[1] * E.nslot[2]);
+    if (E.nslot[0] > maxslots[0])
+        return fail(ICEMPM_ERR_INVALID_ARG, "too many scenes for 1024 nodes per axis of the stored grid");
+    long long nbins = 0;
+    for (int i = 0; i < n_scenes; i++) {
+        E.prms.push_back(scenes[i]);
+        E.ind.push_back(reference_indenter(scenes[i]));
+        const int sl[3] = {i / (E.nslot[1] * E.nslot[2]), (i / E.nslot[2]) % E.nslot[1], i % E.nslot[2]};
+        for (int d = 0; d < 3; d++) E.origin.push_back(sl[d] * E.slot_b[d] * 4);
+        E.bins_off.push_back(nbins);
+        nbins += (long long)scenes[i].IndenterSubdivisions * scenes[i].GridZ * 3;
+    }
+    ctx->ens = E;
+    icempm_SimParams3D stored = scenes[0];
+    stored.GridXTotal = E.slot_b[0] * E.nslot[0] * 4;
+    stored.GridY = E.slot_b[1] * E.nslot[1] * 4;
+    stored.GridZ = E.slot_b[2] * E.nslot[2] * 4;
+    stored.DomainDimensionX = stored.GridXTotal * stored.cellsize;
+    ICEMPM_TRY(alloc_common(ctx, stored, n_points_capacity, nbins));
+    ICEMPM_TRY(dev_alloc(&ctx->ens.d_slots, (size_t)E.nslot[0] * E.nslot[1] * E.nslot[2], "ensemble slots"));
+    ICEMPM_TRY(ens_upload_slots(ctx));
+    fill_kparams(ctx);
+    ctx->have_params = true;
+    return clear_err(ctx);
+}
+
+int icempm_ensemble_set_indenter(icempm_ctx *ctx, int scene, const icempm_Indenter *ind) {
+    if (!ctx || !ind) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (ctx->ens.n == 0) return fail(ICEMPM_ERR_NOT_READY, "icempm_allocate_ensemble has not been called");
+    if (scene < 0 || scene >= ctx->ens.n) return fail(ICEMPM_ERR_INVALID_ARG, "scene out of range");
+    if (ind->shape != ICEMPM_INDENTER_CYLINDER_Z && ind->shape != ICEMPM_INDENTER_SPHERE)
+        return fail(ICEMPM_ERR_INVALID_ARG, "unknown indenter shape");
+    ICEMPM_TRY(set_device(ctx));
+    ctx->ens.ind[scene] = *ind;
+    return ens_upload_slots(ctx);
+}
+
+int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long long *bins_offset_out) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    if (ctx->ens.n == 0) return fail(ICEMPM_ERR_NOT_READY, "icempm_allocate_ensemble has not been called");
+    if (scene < 0 || scene >= ctx->ens.n) return fail(ICEMPM_ERR_INVALID_ARG, "scene out of range");
+    for (int d = 0; d < 3; d++)
+        if (origin_out) origin_out[d] = ctx->ens.origin[3 * scene + d];
+    if (bins_offset_out) *bins_offset_out = ctx->ens.bins_off[scene];
+    return ICEMPM_OK;
+}
+
 int icempm_update_constants(icempm_ctx *ctx, const icempm_SimParams3D *prms) {
     if (!ctx || !prms) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
     if (!ctx->allocated) return fail(ICEMPM_ERR_NOT_READY, "icempm_allocate has not been called");
+    if (ctx->ens.n > 0) return fail(ICEMPM_ERR_INVALID_ARG, "an ensemble takes its parameters at allocation");
     ICEMPM_TRY(set_device(ctx));
     const icempm_SimParams3D &old = ctx->prms;
     if (prms->GridXTotal != old.GridXTotal || prms->GridY != old.GridY || prms->GridZ != old.GridZ ||
@@ -494,6 +660,7 @@ int icempm_update_constants(icempm_ctx *ctx, const icempm_SimParams3D *prms) {
 
 int icempm_set_indenter(icempm_ctx *ctx, const icempm_Indenter *ind) {
     if (!ctx || !ind) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (ctx->ens.n > 0) return fail(ICEMPM_ERR_INVALID_ARG, "ensembles set indenters per scene");
     if (ind->shape != ICEMPM_INDENTER_CYLINDER_Z && ind->shape != ICEMPM_INDENTER_SPHERE)
         return fail(ICEMPM_ERR_INVALID_ARG, "unknown indenter shape");
     ctx->ind = *ind;
@@ -502,9 +669,57 @@ int icempm_set_indenter(icempm_ctx *ctx, const icempm_Indenter *ind) {
     return ICEMPM_OK;
 }
 
+namespace {
+int upload_points(icempm_ctx *ctx, const double *hssoa, unsigned capacity, unsigned offset, int n_points);
+}
+
 int icempm_transfer_points_to_device(icempm_ctx *ctx, const double *hssoa, unsigned capacity, unsigned offset,
                                      int n_points) {
     ICEMPM_TRY(check_ready(ctx));
+    if (ctx->ens.n > 0) return fail(ICEMPM_ERR_INVALID_ARG, "ensembles upload with icempm_ensemble_transfer_points");
+    return upload_points(ctx, hssoa, capacity, offset, n_points);
+}
+
+int icempm_ensemble_transfer_points(icempm_ctx *ctx, const double *hssoa, unsigned capacity,
+                                    const int *scene_first) {
+    ICEMPM_TRY(check_ready(ctx));
+    auto &E = ctx->ens;
+    if (E.n == 0) return fail(ICEMPM_ERR_NOT_READY, "icempm_allocate_ensemble has not been called");
+    if (!scene_first || scene_first[0] != 0) return fail(ICEMPM_ERR_INVALID_ARG, "scene_first must start at 0");
+    for (int i = 0; i < E.n; i++)
+        if (scene_first[i + 1] < scene_first[i]) return fail(ICEMPM_ERR_INVALID_ARG, "scene_first must not decrease");
+    const int n = scene_first[E.n];
+    if ((unsigned)n > capacity) return fail(ICEMPM_ERR_CAPACITY, "scene_first exceeds the host capacity");
+    if (n > 0 && !hssoa) return fail(ICEMPM_ERR_INVALID_ARG, "bad host buffer");
+    // every point must start inside its own scene's grid box: a point placed
+    // in another scene's slot would be advanced as a point of that scene
+    for (int i = 0; i < E.n; i++) {
+        const double ext[3] = {E.prms[i].GridXTotal * E.prms[i].cellsize, E.prms[i].GridY * E.prms[i].cellsize,
+                               E.prms[i].GridZ * E.prms[i].cellsize};
+        for (int d = 0; d < 3; d++) {
+            const double *line = hssoa + (size_t)(ICEMPM_POSX + d) * capacity;
+            for (int k = scene_first[i]; k < scene_first[i + 1]; k++)
+                if (!(line[k] >= 0.0 && line[k] <= ext[d]))
+                    return fail(ICEMPM_ERR_INVALID_ARG, "point " + std::to_string(k) + " lies outside the grid of scene " +
+                                                            std::to_string(i));
+        }
+    }
+    // scene-local positions -> stored-grid positions
+    std::vector<double> tmp((size_t)std::max(n, 1) * ICEMPM_N_PTS_ARRAYS);
+    for (int l = 0; l < ICEMPM_N_PTS_ARRAYS; l++)
+        std::memcpy(tmp.data() + (size_t)l * n, hssoa + (size_t)l * capacity, sizeof(double) * n);
+    for (int i = 0; i < E.n; i++)
+        for (int d = 0; d < 3; d++) {
+            const double off = E.origin[3 * i + d] * E.prms[i].cellsize;
+            double *line = tmp.data() + (size_t)(ICEMPM_POSX + d) * n;
+            for (int k = scene_first[i]; k < scene_first[i + 1]; k++) line[k] += off;
+        }
+    E.first.assign(scene_first, scene_first + E.n + 1);
+    return upload_points(ctx, tmp.data(), (unsigned)n, 0, n);
+}
+
+namespace {
+int upload_points(icempm_ctx *ctx, const double *hssoa, unsigned capacity, unsigned offset, int n_points) {
     if (n_points < 0 || (n_points > 0 && !hssoa)) return fail(ICEMPM_ERR_INVALID_ARG, "bad host buffer");
     if (n_points > ctx->capacity) return fail(ICEMPM_ERR_CAPACITY, "more points than the device capacity");
     if ((unsigned long long)offset + (unsigned)n_points > capacity)
@@ -520,6 +735,7 @@ int icempm_transfer_points_to_device(icempm_ctx *ctx, const double *hssoa, unsig
     ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
     return ICEMPM_OK;
 }
+} // namespace
 
 int icempm_transfer_from_device(icempm_ctx *ctx, double *hssoa, unsigned capacity, unsigned offset,
                                 int *n_points_out, double *indenter_bins) {
@@ -540,6 +756,15 @@ int icempm_transfer_from_device(icempm_ctx *ctx, double *hssoa, unsigned capacit
     int code = 0;
     ICEMPM_HIP(hipMemcpyAsync(&code, ctx->err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    const auto &E = ctx->ens;
+    if (E.n > 0 && (int)E.first.size() == E.n + 1 && E.first[E.n] == n) {
+        for (int i = 0; i < E.n; i++)
+            for (int d = 0; d < 3; d++) {
+                const double off = E.origin[3 * i + d] * E.prms[i].cellsize;
+                double *line = hssoa + (size_t)(ICEMPM_POSX + d) * capacity + offset;
+                for (int k = E.first[i]; k < E.first[i + 1]; k++) line[k] -= off;
+            }
+    }
     if (n_points_out) *n_points_out = n;
     if (code) return fail(ICEMPM_ERR_DEVICE_FLAG, "device error indicator " + std::to_string(code));
     return ICEMPM_OK;
@@ -562,7 +787,7 @@ int icempm_p2g(icempm_ctx *ctx) {
 
 int icempm_update_nodes(icempm_ctx *ctx, double cx, double cy, double cz) {
     ICEMPM_TRY(check_ready(ctx));
-    return do_update(ctx, ctx->g, cx, cy, cz);
+    return do_update(ctx, ctx->g, cx, cy, cz, 0.0);
 }
 
 int icempm_g2p(icempm_ctx *ctx, int record_pq) {
@@ -589,7 +814,7 @@ int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int si
         indenter_center(ctx, t, c);
         ICEMPM_TRY(do_reset_grid(ctx, ctx->g));
         ICEMPM_TRY(do_p2g(ctx, ctx->g));
-        ICEMPM_TRY(do_update(ctx, ctx->g, c[0], c[1], c[2]));
+        ICEMPM_TRY(do_update(ctx, ctx->g, c[0], c[1], c[2], t));
         // then G2P of substep k fused with P2G of substep k+1 on the other grid
         for (int k = 0; k + 1 < run; k++) {
             const int nxt = ctx->g ^ 1;
@@ -597,7 +822,7 @@ int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int si
             ICEMPM_TRY(do_g2p2g(ctx, ctx->g, nxt, ((simulation_step + s + k) % every) == 0 ? 1 : 0));
             t += ctx->prms.InitialTimeStep;
             indenter_center(ctx, t, c);
-            ICEMPM_TRY(do_update(ctx, nxt, c[0], c[1], c[2]));
+            ICEMPM_TRY(do_update(ctx, nxt, c[0], c[1], c[2], t));
             ctx->g = nxt;
         }
         ICEMPM_TRY(do_g2p(ctx, ctx->g, ((simulation_step + s + run - 1) % every) == 0 ? 1 : 0));
@@ -638,6 +863,7 @@ int icempm_point_count(icempm_ctx *ctx, int *n) {
 
 int icempm_remove_disabled_and_sort(icempm_ctx *ctx, int *n_points_out) {
     ICEMPM_TRY(check_ready(ctx));
+    if (ctx->ens.n > 0) return fail(ICEMPM_ERR_INVALID_ARG, "an ensemble keeps its scenes' point ranges");
     const int n = ctx->n;
     if (n > 0) {
         PhaseTimer tm(ctx, ICEMPM_PHASE_SORT);

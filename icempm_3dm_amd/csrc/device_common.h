@@ -19,8 +19,9 @@ constexpr int kBrickNodes = 64;
 // so that every context carries its own constants (the reference keeps one
 // __constant__ SimParams3D per device, mpm_kernels.cuh:18).
 struct KParams {
-    int gx, gy, gz;    // GridXTotal, GridY, GridZ (nodes)
-    int nbx, nby, nbz; // bricks per axis
+    int gx, gy, gz;    // GridXTotal, GridY, GridZ of the scene (nodes)
+    int ox, oy, oz;    // the scene's node (0,0,0) in the stored grid (0 for one scene)
+    int nbx, nby, nbz; // bricks per axis of the stored grid
     long long grid_pitch;
 
     double h, h_inv, dt;
@@ -38,7 +39,50 @@ struct KParams {
     int ind_shape, ind_subdiv;
     double ind_rsq;
     double ind_vel[3];
+    double ind_c0[3];   // indenter center at t = 0 in stored-grid coordinates (ensembles)
+    long long bins_off; // the scene's first indenter accumulator
+
+    // Ensembles: independent scenes, each in its own slot of the stored grid
+    // (slots are whole bricks, with at least one empty brick between scene
+    // boxes, so no stencil crosses from one scene to another). ens_n = 0 for
+    // one scene. A brick's scene block is ens_slots[slot of the brick].
+    int ens_n;
+    int ens_slot_b[3]; // slot size in bricks
+    int ens_nslot[3];  // slots per axis
+    const KParams *ens_slots;
 };
+
+// Parameter block of the scene that owns grid brick b (ensembles).
+__device__ __forceinline__ KParams scene_of_brick(const KParams &kp, long long b) {
+    const int bz = (int)(b % kp.nbz);
+    const int by = (int)((b / kp.nbz) % kp.nby);
+    const int bx = (int)(b / ((long long)kp.nbz * kp.nby));
+    const int slot = ((bx / kp.ens_slot_b[0]) * kp.ens_nslot[1] + by / kp.ens_slot_b[1]) * kp.ens_nslot[2] +
+                     bz / kp.ens_slot_b[2];
+    return kp.ens_slots[slot];
+}
+
+// The scene block for work on brick b: kp itself for one scene.
+template <bool kEns>
+__device__ __forceinline__ KParams scene_view(const KParams &kp, long long b) {
+    if constexpr (kEns) return scene_of_brick(kp, b);
+    else return kp;
+}
+
+// The scene block of a point at pos (ensembles; the point's base node picks
+// the brick, clamped into the stored grid).
+__device__ __forceinline__ KParams scene_of_point(const KParams &kp, const double pos[3]) {
+    const int nb[3] = {kp.nbx, kp.nby, kp.nbz};
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double x = pos[d] * kp.h_inv - 0.5;
+        if (!(x >= 0.0)) x = 0.0;
+        if (x > (double)(nb[d] * 4 - 1)) x = (double)(nb[d] * 4 - 1);
+        c[d] = (int)x >> 2;
+    }
+    return scene_of_brick(kp, ((long long)c[0] * kp.nby + c[1]) * kp.nbz + c[2]);
+}
 
 // Particle storage: 28 SoA lines of `pitch` doubles (HostSideSOA line order).
 struct PtsView {
@@ -63,7 +107,8 @@ __device__ __forceinline__ long long double_to_bits(double d) { return __double_
 __device__ __forceinline__ void raise_error(int *err, int code) { atomicMax(err, code); }
 
 // Base node, fractional offset and error code for a particle position
-// (mpm_kernels.cuh:65-75). Returns 0 if the 3x3x3 stencil lies in the grid.
+// (mpm_kernels.cuh:65-75). Returns 0 if the 3x3x3 stencil lies in the
+// scene's grid (nodes [o, o + g) per axis).
 __device__ __forceinline__ int locate(const KParams &kp, const double pos[3], int base[3], double fx[3]) {
     double xi[3];
 #pragma unroll
@@ -78,10 +123,11 @@ __device__ __forceinline__ int locate(const KParams &kp, const double pos[3], in
     }
     int code = 0;
     const int g[3] = {kp.gx, kp.gy, kp.gz};
+    const int o[3] = {kp.ox, kp.oy, kp.oz};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        if (base[d] < 0) code = max(code, 70 + d);
-        if (base[d] > g[d] - 3) code = max(code, 73 + d);
+        if (base[d] < o[d]) code = max(code, 70 + d);
+        if (base[d] > o[d] + g[d] - 3) code = max(code, 73 + d);
     }
     return code;
 }

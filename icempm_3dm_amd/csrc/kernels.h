@@ -51,7 +51,7 @@ hipError_t launch_reset_grid(const KParams &kp, GridState gs, int cur, hipStream
 // Zeroes the whole grid, all flags and both counts (after allocation).
 hipError_t launch_clear_grid_all(const KParams &kp, GridState gs, long long nbricks, hipStream_t stream);
 // partition_kernel_update_nodes over the bricks of list `cur`.
-hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double cx, double cy, double cz,
+hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double cx, double cy, double cz, double t,
                                double *bins, hipStream_t stream);
 // Copy into the reference's x-major node order.
 hipError_t launch_grid_to_xmajor(const KParams &kp, const double *grid, double *out, hipStream_t stream);

@@ -17,6 +17,7 @@ __global__ void k_zero_count(int *c) { *c = 0; }
 // drifted outside its segment's tile since the sort goes to the spill list.
 constexpr int kG2PWaves = kP2GThreads / 64;
 
+template <bool kEns>
 __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, SegInfo seg,
                                                      const double *__restrict__ grid, int record_pq, int *err,
                                                      int *spill, int *spill_count) {
@@ -29,11 +30,12 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, Se
     const double c4h = 4. * kp.h_inv;
     for (long long sg = nseg * w / nw, sg_hi = nseg * (w + 1) / nw; sg < sg_hi; sg++) {
         const int begin = seg.seg_start[sg], end = seg.seg_start[sg + 1];
+        const KParams ks = scene_view<kEns>(kp, seg.seg_brick[sg]);
         int org[3], bc[3];
-        tile_origin(kp, seg.seg_brick[sg], org, bc);
+        tile_origin(ks, seg.seg_brick[sg], org, bc);
         // the previous segment's gathers precede these writes
         wave_lds_sync();
-        vel_tile_load_wave(sv, lane, kp, org, grid);
+        vel_tile_load_wave(sv, lane, ks, org, grid);
         wave_lds_sync();
         for (int c0 = begin; c0 < end; c0 += 64) {
             const int i = c0 + lane;
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, Se
             if (!(s.util & ICEMPM_STATUS_DISABLED)) {
                 int base[3];
                 double fx[3];
-                const int code = locate(kp, s.pos, base, fx);
+                const int code = locate(ks, s.pos, base, fx);
                 if (code) {
                     raise_error(err, code);
                 } else {
@@ -53,7 +55,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, Se
                         double w3[3][3], d[3][3];
                         stencil_weights(fx, w3, d);
                         gather_tile(sv, tile_index(rx, ry, rz), w3, d, c4h, s.vel, s.Bp);
-                        g2p_finish<true>(kp, pts, i, s, record_pq);
+                        g2p_finish<true>(ks, pts, i, s, record_pq);
                     } else {
                         spilled = true;
                     }
@@ -73,13 +75,14 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p_spill(KParams kp, PtsView p
         const int i = spill[k];
         PointState s;
         load_g2p_inputs(pts, i, s);
+        const KParams ks = kp.ens_n > 0 ? scene_of_point(kp, s.pos) : kp;
         int base[3];
         double fx[3];
-        (void)locate(kp, s.pos, base, fx); // listed points passed the bounds check
+        (void)locate(ks, s.pos, base, fx); // listed points passed the bounds check
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
-        gather_global(kp, grid, base, w, d, c4h, s.vel, s.Bp);
-        g2p_finish<true>(kp, pts, i, s, record_pq);
+        gather_global(ks, grid, base, w, d, c4h, s.vel, s.Bp);
+        g2p_finish<true>(ks, pts, i, s, record_pq);
     }
 }
 
@@ -93,7 +96,9 @@ hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, SegInfo seg, const 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
-    k_g2p<<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, err, spill, spill_count);
+    if (kp.ens_n > 0)
+        k_g2p<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, err, spill, spill_count);
+    else k_g2p<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, err, spill, spill_count);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_g2p_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid, record_pq, spill, spill_count);
     return hipGetLastError();

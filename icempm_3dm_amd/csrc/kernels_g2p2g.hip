@@ -22,6 +22,7 @@ __global__ void k_zero_count(int *c) { *c = 0; }
 // LDS (two tiles and the records, about 79 KB) already limits the kernel to
 // two workgroups per CU; the bound keeps the compiler within the matching
 // 256 registers instead of spilling past them into AGPRs.
+template <bool kEns>
 __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
                                                        const double *__restrict__ grid_a, GridState gs_b, int cur_b,
                                                        int record_pq, int *err, int *spill, int *spill_count) {
@@ -37,12 +38,13 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
     pf.util = ICEMPM_STATUS_DISABLED;
     if (c.valid() && c.c0 + t < c.end) load_g2p_inputs(pts, c.c0 + t, pf);
     while (c.valid()) {
+        const KParams ks = scene_view<kEns>(kp, c.brick);
         int org[3], bc[3];
-        tile_origin(kp, c.brick, org, bc);
+        tile_origin(ks, c.brick, org, bc);
         if (c.first()) {
             // the previous segment's gathers and flush are finished (barrier at its end)
             VelTileRegs vt;
-            vel_tile_fetch(vt, t, kp, org, grid_a);
+            vel_tile_fetch(vt, t, ks, org, grid_a);
             vel_tile_store(sv, vt, t);
             p2g_zero_tile(sh, t);
             __syncthreads();
@@ -57,7 +59,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         if (!(s.util & ICEMPM_STATUS_DISABLED)) {
             int base[3];
             double fx[3];
-            const int code = locate(kp, s.pos, base, fx);
+            const int code = locate(ks, s.pos, base, fx);
             if (code) {
                 raise_error(err, code);
             } else {
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
                     double w[3][3], d[3][3];
                     stencil_weights(fx, w, d);
                     gather_tile(sv, tile_index(rx, ry, rz), w, d, c4h, s.vel, s.Bp);
-                    g2p_finish<true>(kp, pts, i, s, record_pq);
+                    g2p_finish<true>(ks, pts, i, s, record_pq);
                     updated = true;
                 } else {
                     spilled = true;
@@ -76,12 +78,12 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         wave_append(spilled, i, spill, spill_count);
         // P2G record of the new state, straight from registers
         const double Je = updated ? det3(s.Fe) : 1.0;
-        p2g_record(sh, t, kp, updated, s.util, s.pos, s.vel, s.Fe, Je, rcbrt(Je * Je), s.Bp, s.Jp, org, err);
+        p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rcbrt(Je * Je), s.Bp, s.Jp, org, err);
         __syncthreads();
         pf.util = ICEMPM_STATUS_DISABLED;
         if (nx.valid() && nx.c0 + t < nx.end) load_g2p_inputs(pts, nx.c0 + t, pf);
-        p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), kp, gs_b, bl);
-        if (c.last()) p2g_flush(sh, t, kp, org, bc, gs_b, bl);
+        p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
+        if (c.last()) p2g_flush(sh, t, ks, org, bc, gs_b, bl);
         c = nx;
     }
 }
@@ -99,14 +101,15 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
         const int i = spill[k];
         PointState s;
         load_g2p_inputs(pts, i, s);
+        const KParams ks = kp.ens_n > 0 ? scene_of_point(kp, s.pos) : kp;
         int base[3];
         double fx[3];
-        (void)locate(kp, s.pos, base, fx); // listed points passed the bounds check
+        (void)locate(ks, s.pos, base, fx); // listed points passed the bounds check
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
-        gather_global(kp, grid_a, base, w, d, c4h, s.vel, s.Bp);
-        g2p_finish<true>(kp, pts, i, s, record_pq);
-        const int code = scatter_point_global(kp, gs_b, bl, s);
+        gather_global(ks, grid_a, base, w, d, c4h, s.vel, s.Bp);
+        g2p_finish<true>(ks, pts, i, s, record_pq);
+        const int code = scatter_point_global(ks, gs_b, bl, s);
         if (code) raise_error(err, code);
     }
 }
@@ -121,8 +124,12 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
-    k_g2p2g<<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, err, spill,
-                                                  spill_count);
+    if (kp.ens_n > 0)
+        k_g2p2g<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, err, spill,
+                                                            spill_count);
+    else
+        k_g2p2g<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, err,
+                                                             spill, spill_count);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_g2p2g_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid_a, gs_b, cur_b, record_pq, err, spill, spill_count);
     return hipGetLastError();

@@ -41,25 +41,38 @@ __global__ void k_clear_all(KParams kp, GridState gs, long long nbricks) {
 
 // partition_kernel_update_nodes (mpm_kernels.cuh:138-232) for the nodes of
 // the listed bricks; the indenter is the reference cylinder along z or a sphere.
-__global__ __launch_bounds__(kThreads) void k_update_nodes(KParams kp, GridState gs, int cur, double cx,
-                                                           double cy, double cz, double *bins) {
+// One scene: the indenter center is (cx, cy, cz). Ensembles: each scene's
+// center is its c0 + t * velocity, walls and force bins are the scene's.
+template <bool kEns>
+__global__ __launch_bounds__(kThreads) void k_update_nodes(KParams kp0, GridState gs, int cur, double cx0,
+                                                           double cy0, double cz0, double t, double *bins0) {
     const int count = gs.brick_count[cur];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int32_t *list = gs.brick_list[cur];
-    const double dt = kp.dt;
-    const double cellsize = kp.h;
-    const bool sphere = kp.ind_shape == ICEMPM_INDENTER_SPHERE;
     for (int k = blockIdx.x * kWavesPerBlock + wave; k < count; k += gridDim.x * kWavesPerBlock) {
         const int b = list[k];
         const long long idx = (long long)b * kBrickNodes + lane;
         const double mass = gs.grid[idx];
         if (mass == 0) continue;
+        const KParams kp = scene_view<kEns>(kp0, b);
+        const double dt = kp.dt;
+        const double cellsize = kp.h;
+        const bool sphere = kp.ind_shape == ICEMPM_INDENTER_SPHERE;
+        double cx = cx0, cy = cy0, cz = cz0;
+        if constexpr (kEns) {
+            cx = kp.ind_c0[0] + t * kp.ind_vel[0];
+            cy = kp.ind_c0[1] + t * kp.ind_vel[1];
+            cz = kp.ind_c0[2] + t * kp.ind_vel[2];
+        }
+        double *bins = bins0 + kp.bins_off;
         const int bz = b % kp.nbz;
         const int by = (b / kp.nbz) % kp.nby;
         const int bx = b / (kp.nbz * kp.nby);
-        const int gi[3] = {bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
-        const double gnpos[3] = {gi[0] * cellsize, gi[1] * cellsize, gi[2] * cellsize};
+        const int gs_[3] = {bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
+        const double gnpos[3] = {gs_[0] * cellsize, gs_[1] * cellsize, gs_[2] * cellsize};
+        // node index within the scene's grid (walls, sticky band, force bins)
+        const int gi[3] = {gs_[0] - kp.ox, gs_[1] - kp.oy, gs_[2] - kp.oz};
 
         double v[3];
 #pragma unroll
@@ -152,9 +165,10 @@ hipError_t launch_clear_grid_all(const KParams &kp, GridState gs, long long nbri
     return hipGetLastError();
 }
 
-hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double cx, double cy, double cz,
+hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double cx, double cy, double cz, double t,
                                double *bins, hipStream_t stream) {
-    k_update_nodes<<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, bins);
+    if (kp.ens_n > 0) k_update_nodes<true><<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, t, bins);
+    else k_update_nodes<false><<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, t, bins);
     return hipGetLastError();
 }
 

@@ -33,6 +33,7 @@ __device__ __forceinline__ void load_p2g_inputs(PtsView pts, int i, bool valid, 
     if (r.util & ICEMPM_STATUS_LIQUID) r.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
 }
 
+template <bool kEns>
 __global__ __launch_bounds__(kP2GThreads) void k_p2g(KParams kp, PtsView pts, SegInfo seg, GridState gs, int cur,
                                                      int *err) {
     __shared__ P2GShared sh;
@@ -43,17 +44,18 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(KParams kp, PtsView pts, Se
     P2GInputs in;
     if (c.valid()) load_p2g_inputs(pts, c.c0 + t, c.c0 + t < c.end, in);
     while (c.valid()) {
+        const KParams ks = scene_view<kEns>(kp, c.brick);
         int org[3], bc[3];
-        tile_origin(kp, c.brick, org, bc);
+        tile_origin(ks, c.brick, org, bc);
         if (c.first()) p2g_zero_tile(sh, t);
         const double Je = (in.util & ICEMPM_STATUS_DISABLED) ? 1.0 : det3(in.Fe);
-        p2g_record(sh, t, kp, c.c0 + t < c.end, in.util, in.pos, in.vel, in.Fe, Je, rcbrt(Je * Je), in.Bp, in.Jp,
+        p2g_record(sh, t, ks, c.c0 + t < c.end, in.util, in.pos, in.vel, in.Fe, Je, rcbrt(Je * Je), in.Bp, in.Jp,
                    org, err);
         const ChunkCursor nx = cursor_next(c, seg);
         if (nx.valid()) load_p2g_inputs(pts, nx.c0 + t, nx.c0 + t < nx.end, in);
         __syncthreads();
-        p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), kp, gs, bl);
-        if (c.last()) p2g_flush(sh, t, kp, org, bc, gs, bl);
+        p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs, bl);
+        if (c.last()) p2g_flush(sh, t, ks, org, bc, gs, bl);
         c = nx;
     }
 }
@@ -64,7 +66,8 @@ hipError_t launch_p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, GridSt
                       int n_blocks, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
-    k_p2g<<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, gs, cur, err);
+    if (kp.ens_n > 0) k_p2g<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, gs, cur, err);
+    else k_p2g<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, gs, cur, err);
     return hipGetLastError();
 }
 

@@ -50,8 +50,10 @@ __global__ void k_cell_keys(KParams kp, PtsView pts, int n, int kind, uint32_t *
         int iz = (int)(pts.at(ICEMPM_POSX + 2, i) * kp.h_inv + 0.5);
         key = (uint32_t)(ix * kp.gy * kp.gz + iy * kp.gz + iz);
     } else {
-        // Same nearest-node cell, clamped into the grid, grouped in 4^3 bricks.
-        const int g[3] = {kp.gx, kp.gy, kp.gz};
+        // Same nearest-node cell, clamped into the grid, grouped in 4^3 bricks
+        // (ensembles: the stored grid that holds every scene).
+        const int g[3] = {kp.ens_n > 0 ? kp.nbx * 4 : kp.gx, kp.ens_n > 0 ? kp.nby * 4 : kp.gy,
+                          kp.ens_n > 0 ? kp.nbz * 4 : kp.gz};
         int c[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {

@@ -243,6 +243,37 @@ int icempm_save_snapshot(const char *path, const icempm_SimParams3D *prms, const
 int icempm_read_snapshot(const char *path, icempm_SimParams3D *prms, double *hssoa, unsigned capacity,
                          unsigned *capacity_out, unsigned *size_out, double *indenter_bins, int n_indenter_bins);
 
+/* Batched ensemble (no reference counterpart; the reference runs one scene
+ * per process). n_scenes independent scenes, each with its own SimParams3D
+ * (grid size, material, indenter) but a common InitialTimeStep and cellsize,
+ * advance in the same kernel launches: every scene owns a slot of whole 4^3
+ * node bricks in one stored grid, slots at least one empty brick apart, so no
+ * point of one scene reaches a node of another. A scene's grid, walls,
+ * indenter and accumulators behave as in a single-scene context allocated
+ * with its parameters; a point leaving its scene's grid raises the device
+ * error indicator as there. After this call the context takes
+ * icempm_ensemble_transfer_points, icempm_ensemble_set_indenter,
+ * icempm_step, icempm_reset_indenter_force_accumulator,
+ * icempm_transfer_from_device (points back in scene-local coordinates, the
+ * accumulators of all scenes concatenated in scene order) and
+ * icempm_download_grid (the whole stored grid); icempm_update_constants,
+ * icempm_set_indenter, icempm_transfer_points_to_device and
+ * icempm_remove_disabled_and_sort return ICEMPM_ERR_INVALID_ARG. The output
+ * substep for the p/q record follows scene 0's UpdateEveryNthStep.
+ * ICEMPM_ERR_INVALID_ARG if the slots do not fit 1024 nodes per axis. */
+int icempm_allocate_ensemble(icempm_ctx *ctx, const icempm_SimParams3D *scenes, int n_scenes, int n_points_capacity);
+/* Replace scene `scene`'s indenter law; the center is in scene coordinates. */
+int icempm_ensemble_set_indenter(icempm_ctx *ctx, int scene, const icempm_Indenter *ind);
+/* Upload the points of every scene: scene i owns host points
+ * [scene_first[i], scene_first[i+1]) (scene_first has n_scenes+1 entries,
+ * scene_first[0] = 0), positions in scene coordinates. */
+int icempm_ensemble_transfer_points(icempm_ctx *ctx, const double *hssoa, unsigned capacity,
+                                    const int *scene_first);
+/* Where scene `scene` lives: node offset of its node (0,0,0) in the stored
+ * grid and the offset of its IndenterSubdivisions*GridZ*3 accumulators in the
+ * concatenated array. Either output may be NULL. */
+int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long long *bins_offset_out);
+
 /* Synthetic input (no reference counterpart; replaces the HDF5 point cloud of
  * SnapshotManager::LoadRawPoints for generated scenes): a block of
  * cells[0]*cells[1]*cells[2] lattice cells of edge `cell_size` starting at
