@@ -376,11 +376,13 @@ int do_g2p(icempm_ctx *ctx, int gi, int record_pq) {
 }
 
 // G2P of one substep from grid ga fused with P2G of the next into grid gb.
+// Only icempm_step calls it, always followed by another fused pass or the
+// closing G2P, which recompute velocity and C: those lines are not stored.
 int do_g2p2g(icempm_ctx *ctx, int ga, int gb, int record_pq) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_G2P2G);
     SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
     ICEMPM_HIP(launch_g2p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[ga].grid, ctx->gstate(gb),
-                            ctx->grids[gb].cur, record_pq, ctx->err, ctx->work, ctx->p2g_blocks, ctx->stream));
+                            ctx->grids[gb].cur, record_pq, 0, ctx->err, ctx->work, ctx->p2g_blocks, ctx->stream));
     ctx->substeps_since_sort++;
     return tm.finish();
 }

@@ -23,6 +23,15 @@ __device__ __forceinline__ void load_g2p_inputs(PtsView pts, int i, PointState &
     for (int a = 0; a < 3; a++) s.pos[a] = pts.at(ICEMPM_POSX + a, i);
 }
 
+// All 14 doubles G2P reads of a point, for a caller that loads them a chunk
+// ahead (g2p_finish<..., true> then skips its own Fe and Jp_inv loads).
+__device__ __forceinline__ void load_g2p_state(PtsView pts, int i, PointState &s) {
+    load_g2p_inputs(pts, i, s);
+#pragma unroll
+    for (int k = 0; k < 9; k++) s.Fe[k] = pts.at(ICEMPM_FE00 + k, i);
+    s.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
+}
+
 // Node velocities of a tile: values a thread fetches for nodes t and t+256
 // (fetched ahead, stored to LDS later).
 struct VelTileRegs {
@@ -181,28 +190,33 @@ __device__ __forceinline__ void gather_global(const KParams &kp, const double *g
 
 // Everything after the gather (mpm_kernels.cuh:307-351): reads Fe and
 // Jp_inv, then advection, F update, p/q, crush test; stores the 26 (+2 with
-// record_pq) lines and leaves the new state in s. With kInlineReturnMap the
+// record_pq) lines and leaves the new state in s. Without store_vc the
+// velocity and C lines are not stored: a caller whose next reader of the
+// point is another G2P (which recomputes both from the grid) saves the 96 B. With kInlineReturnMap the
 // return mapping of a crushed point runs here and false is returned;
 // otherwise the trial F is stored and true is returned for a crushed solid
 // point, whose return mapping (return_map_point) must follow before its F
-// is used.
-template <bool kInlineReturnMap>
-__device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i, PointState &s, int record_pq) {
+// is used. With kPreloaded, s.Fe and s.Jp already hold the stored values.
+template <bool kInlineReturnMap, bool kPreloaded = false>
+__device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i, PointState &s, int record_pq,
+                                           bool store_vc = true) {
     const double dt = kp.dt;
     const bool liquid = s.util & ICEMPM_STATUS_LIQUID;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         s.pos[a] += s.vel[a] * dt;
         pts.at(ICEMPM_POSX + a, i) = s.pos[a];
-        pts.at(ICEMPM_VELX + a, i) = s.vel[a];
+        if (store_vc) pts.at(ICEMPM_VELX + a, i) = s.vel[a];
     }
+    if (!kPreloaded) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) s.Fe[k] = pts.at(ICEMPM_FE00 + k, i);
-    s.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
+        for (int k = 0; k < 9; k++) s.Fe[k] = pts.at(ICEMPM_FE00 + k, i);
+        s.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
+    }
     double A[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        pts.at(ICEMPM_BP00 + k, i) = s.Bp[k];
+        if (store_vc) pts.at(ICEMPM_BP00 + k, i) = s.Bp[k];
         A[k] = dt * s.Bp[k] + ((k % 4 == 0) ? 1.0 : 0.0);
     }
     PState p;

@@ -69,8 +69,10 @@ hipError_t launch_p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, GridSt
 hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid, int record_pq, int *err,
                       int *lists, int n_blocks, hipStream_t stream);
 // G2P of one substep reading grid_a fused with P2G of the next substep into
-// gs_b (kernels_g2p2g.hip). lists: 1 + n ints of scratch.
+// gs_b (kernels_g2p2g.hip). lists: 1 + n ints of scratch. store_vc = 0
+// leaves the velocity and C lines unwritten (the next reader is a G2P).
 hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid_a, GridState gs_b,
-                        int cur_b, int record_pq, int *err, int *lists, int n_blocks, hipStream_t stream);
+                        int cur_b, int record_pq, int store_vc, int *err, int *lists, int n_blocks,
+                        hipStream_t stream);
 
 } // namespace icempm

@@ -9,8 +9,12 @@
 //      workgroup scatters the chunk into grid B (LDS tile, flushed with
 //      atomics at the end of the segment).
 // Per point that is 112 B read + 208 B written to HBM instead of 200 + 112
-// + 208 B for the two separate passes. Points outside their segment's tile
-// at the start of the substep go to a spill list (k_g2p2g_spill).
+// + 208 B for the two separate passes. When the next reader of the points is
+// again a G2P (store_vc = 0: another fused pass or the closing G2P of the
+// substep loop), velocity and C are not stored either (they are recomputed
+// from the grid there): 112 B read + 112 B written. Points outside their
+// segment's tile at the start of the substep go to a spill list
+// (k_g2p2g_spill).
 #include "g2p_point.h"
 
 namespace icempm {
@@ -25,7 +29,8 @@ __global__ void k_zero_count(int *c) { *c = 0; }
 template <bool kEns>
 __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
                                                        const double *__restrict__ grid_a, GridState gs_b, int cur_b,
-                                                       int record_pq, int *err, int *spill, int *spill_count) {
+                                                       int record_pq, int store_vc, int *err, int *spill,
+                                                       int *spill_count) {
     __shared__ P2GShared sh;
     __shared__ double sv[3][kTileAlloc];
     const int t = threadIdx.x;
@@ -33,10 +38,10 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
     const int nseg = *seg.nseg;
     const double c4h = 4. * kp.h_inv;
     ChunkCursor c = cursor_first(seg, nseg, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
-    // utility and position of the next chunk are loaded during the scatter
+    // what G2P reads of the next chunk's points is loaded during the scatter
     PointState pf;
     pf.util = ICEMPM_STATUS_DISABLED;
-    if (c.valid() && c.c0 + t < c.end) load_g2p_inputs(pts, c.c0 + t, pf);
+    if (c.valid() && c.c0 + t < c.end) load_g2p_state(pts, c.c0 + t, pf);
     while (c.valid()) {
         const KParams ks = scene_view<kEns>(kp, c.brick);
         int org[3], bc[3];
@@ -54,6 +59,9 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         s.util = pf.util;
 #pragma unroll
         for (int a = 0; a < 3; a++) s.pos[a] = pf.pos[a];
+#pragma unroll
+        for (int k = 0; k < 9; k++) s.Fe[k] = pf.Fe[k];
+        s.Jp = pf.Jp;
         const ChunkCursor nx = cursor_next(c, seg);
         bool updated = false, spilled = false;
         if (!(s.util & ICEMPM_STATUS_DISABLED)) {
@@ -68,7 +76,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
                     double w[3][3], d[3][3];
                     stencil_weights(fx, w, d);
                     gather_tile(sv, tile_index(rx, ry, rz), w, d, c4h, s.vel, s.Bp);
-                    g2p_finish<true>(ks, pts, i, s, record_pq);
+                    g2p_finish<true, true>(ks, pts, i, s, record_pq, store_vc != 0);
                     updated = true;
                 } else {
                     spilled = true;
@@ -81,7 +89,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rcbrt(Je * Je), s.Bp, s.Jp, org, err);
         __syncthreads();
         pf.util = ICEMPM_STATUS_DISABLED;
-        if (nx.valid() && nx.c0 + t < nx.end) load_g2p_inputs(pts, nx.c0 + t, pf);
+        if (nx.valid() && nx.c0 + t < nx.end) load_g2p_state(pts, nx.c0 + t, pf);
         p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
         if (c.last()) p2g_flush(sh, t, ks, org, bc, gs_b, bl);
         c = nx;
@@ -92,8 +100,8 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
 // of the new state straight into grid B.
 __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView pts,
                                                              const double *__restrict__ grid_a, GridState gs_b,
-                                                             int cur_b, int record_pq, int *err, const int *spill,
-                                                             const int *spill_count) {
+                                                             int cur_b, int record_pq, int store_vc, int *err,
+                                                             const int *spill, const int *spill_count) {
     const int count = *spill_count;
     const double c4h = 4. * kp.h_inv;
     const BrickList bl = brick_list(gs_b, cur_b);
@@ -108,7 +116,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
         gather_global(ks, grid_a, base, w, d, c4h, s.vel, s.Bp);
-        g2p_finish<true>(ks, pts, i, s, record_pq);
+        g2p_finish<true>(ks, pts, i, s, record_pq, store_vc != 0);
         const int code = scatter_point_global(ks, gs_b, bl, s);
         if (code) raise_error(err, code);
     }
@@ -117,7 +125,8 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
 } // namespace
 
 hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid_a, GridState gs_b,
-                        int cur_b, int record_pq, int *err, int *lists, int n_blocks, hipStream_t stream) {
+                        int cur_b, int record_pq, int store_vc, int *err, int *lists, int n_blocks,
+                        hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     int *spill_count = lists, *spill = lists + 1;
     k_zero_count<<<1, 1, 0, stream>>>(spill_count);
@@ -125,13 +134,14 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
     if (e != hipSuccess) return e;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
     if (kp.ens_n > 0)
-        k_g2p2g<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, err, spill,
-                                                            spill_count);
+        k_g2p2g<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, store_vc,
+                                                            err, spill, spill_count);
     else
-        k_g2p2g<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, err,
-                                                             spill, spill_count);
+        k_g2p2g<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, store_vc,
+                                                             err, spill, spill_count);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_g2p2g_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid_a, gs_b, cur_b, record_pq, err, spill, spill_count);
+    k_g2p2g_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid_a, gs_b, cur_b, record_pq, store_vc, err, spill,
+                                                    spill_count);
     return hipGetLastError();
 }
 

@@ -154,19 +154,27 @@ __device__ __forceinline__ void column_zero(Column &c) {
     for (int k = 0; k < 3; k++) c.m[k] = c.v0[k] = c.v1[k] = c.v2[k] = 0.0;
 }
 
-// Adds record j: W (m, a + Mh o) for o = (ox, oy, k).
+// Adds record j: W (m, a + Mh o) for o = (ox, oy, k). All 18 record values
+// are read before any is used, so one LDS round trip serves the point.
 __device__ __forceinline__ void column_add(const P2GShared &sh, int j, int ox, int oy, double dox, double doy,
                                            Column &c) {
-    const double wxy = sh.rec[1 + ox][j] * sh.rec[4 + oy][j];
+    const double wx = sh.rec[1 + ox][j], wy = sh.rec[4 + oy][j];
     const double m = sh.rec[0][j];
+    const double wz[3] = {sh.rec[7][j], sh.rec[8][j], sh.rec[9][j]};
+    double a[3], M[9];
+#pragma unroll
+    for (int r = 0; r < 3; r++) a[r] = sh.rec[10 + r][j];
+#pragma unroll
+    for (int k = 0; k < 9; k++) M[k] = sh.rec[13 + k][j];
+    __builtin_amdgcn_sched_barrier(0);
+    const double wxy = wx * wy;
     double base[3];
 #pragma unroll
-    for (int r = 0; r < 3; r++)
-        base[r] = sh.rec[10 + r][j] + sh.rec[13 + 3 * r][j] * dox + sh.rec[14 + 3 * r][j] * doy;
-    const double mz[3] = {sh.rec[15][j], sh.rec[18][j], sh.rec[21][j]}; // Mh(r, 2)
+    for (int r = 0; r < 3; r++) base[r] = a[r] + M[3 * r] * dox + M[3 * r + 1] * doy;
+    const double mz[3] = {M[2], M[5], M[8]}; // Mh(r, 2)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double W = wxy * sh.rec[7 + k][j];
+        const double W = wxy * wz[k];
         c.m[k] += W * m;
         const double u0 = (k == 0) ? base[0] : base[0] + mz[0] * (double)k;
         const double u1 = (k == 0) ? base[1] : base[1] + mz[1] * (double)k;
@@ -295,12 +303,38 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
     __syncthreads();
 }
 
+// Tile node of flush slot k, in the address order of the grid's bricked
+// layout: the tile spans 2 + 4 + 2 nodes of three bricks per axis; slots run
+// over the 27 brick pieces (x, then y, then z) and within a piece in the
+// brick's node order. A wavefront touches as many cache lines as in plain
+// x-major slot order, but the flush measured about 2% faster this way.
+__device__ __forceinline__ int piece_start(int p) { return p == 0 ? 0 : (p == 1 ? 2 : 6); }
+__device__ __forceinline__ int piece_size(int p) { return p == 1 ? 4 : 2; }
+__device__ __forceinline__ void flush_order(int k, int &tx, int &ty, int &tz) {
+    const int px = k < 128 ? 0 : (k < 384 ? 1 : 2);
+    const int sx = piece_size(px);
+    int r = k - piece_start(px) * 64;
+    const int py = r < sx * 16 ? 0 : (r < sx * 48 ? 1 : 2);
+    const int sy = piece_size(py);
+    r -= sx * piece_start(py) * 8;
+    const int pz = r < sx * sy * 2 ? 0 : (r < sx * sy * 6 ? 1 : 2);
+    const int sz = piece_size(pz);
+    r -= sx * sy * piece_start(pz);
+    const int syz = sy * sz;
+    const int ix = r / syz, rem = r - ix * syz;
+    const int iy = rem / sz, iz = rem - iy * sz;
+    tx = piece_start(px) + ix;
+    ty = piece_start(py) + iy;
+    tz = piece_start(pz) + iz;
+}
+
 // Adds the tile to the grid and marks the touched bricks (call after a
 // barrier; ends with a barrier).
 __device__ __forceinline__ void p2g_flush(P2GShared &sh, int t, const KParams &kp, const int org[3],
                                           const int bc[3], const GridState &gs, const BrickList &bl) {
     for (int k = t; k < kTileNodes; k += kP2GThreads) {
-        const int tx = k >> 6, ty = (k >> 3) & 7, tz = k & 7;
+        int tx, ty, tz;
+        flush_order(k, tx, ty, tz);
         const int tk = tile_index(tx, ty, tz);
         const double m = sh.tile[0][tk];
         if (m != 0.0) {
