@@ -25,8 +25,9 @@ STATUS_CRUSHED = 0x10000
 STATUS_DISABLED = 0x20000
 STATUS_LIQUID = 0x40000
 
-OK, ERR_INVALID_ARG, ERR_HIP, ERR_NOT_READY, ERR_CAPACITY, ERR_DEVICE_FLAG, ERR_NO_DEVICE = range(7)
+OK, ERR_INVALID_ARG, ERR_HIP, ERR_NOT_READY, ERR_CAPACITY, ERR_DEVICE_FLAG, ERR_NO_DEVICE, ERR_PLASTIC = range(8)
 DEVERR_NONFINITE = 76
+ADJ_PLASTIC, ADJ_UNSUPPORTED = 90, 91
 INDENTER_CYLINDER_Z, INDENTER_SPHERE = 0, 1
 PHASES = ("sort", "reset", "p2g", "update", "g2p", "g2p2g")
 STATS = ("spilled", "sorts", "segments")
@@ -125,6 +126,12 @@ class Indenter(C.Structure):
         return ind
 
 
+class AdjointRun(C.Structure):
+    """icempm_AdjointRun: substeps, force sample length, checkpoint period, start time."""
+
+    _fields_ = [("n_steps", C.c_int), ("sample_every", C.c_int), ("checkpoint_every", C.c_int), ("t0", C.c_double)]
+
+
 class IcempmError(RuntimeError):
     """A C-ABI call returned an error code (the reference throws std::runtime_error)."""
 
@@ -180,6 +187,8 @@ _PROTOS = {
     "icempm_ensemble_set_indenter": (C.c_int, [_ctxp, C.c_int, C.POINTER(Indenter)]),
     "icempm_ensemble_transfer_points": (C.c_int, [_ctxp, _dp, C.c_uint, _ip]),
     "icempm_ensemble_layout": (C.c_int, [_ctxp, C.c_int, _ip, C.POINTER(C.c_longlong)]),
+    "icempm_adjoint_forward": (C.c_int, [_ctxp, C.POINTER(AdjointRun), _dp]),
+    "icempm_adjoint_backward": (C.c_int, [_ctxp, _dp, _dp, _dp]),
     "icempm_scene_block": (C.c_int, [_dp, C.c_uint, _ip, C.c_int, _dp, C.c_double, C.c_double,
                                      C.c_ulonglong, C.c_int, _ip]),
     "icempm_read_raw_points": (C.c_int, [C.c_char_p, _dp, C.c_uint, _ip, _dp]),

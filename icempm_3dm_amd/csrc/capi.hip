@@ -115,6 +115,28 @@ struct icempm_ctx {
         KParams *d_slots = nullptr;
     } ens;
 
+    // calibration adjoint: checkpoints of a forward run, replayed per segment
+    struct Adjoint {
+        int n = 0;
+        icempm_AdjointRun run{};
+        int nseg = 0;
+        double *ckpt = nullptr;  // nseg point states (28 lines of n)
+        double *traj = nullptr;  // K + 1 point states of the segment being replayed
+        double *G = nullptr;     // K grids after P2G (4 lines of grid_pitch)
+        double *V = nullptr;     // K grids after the update
+        double *Gbar = nullptr;  // m_bar, mv_bar (4 lines)
+        double *vbar = nullptr;  // v_bar (4 lines; line 0 unused) ...
+        int32_t *vbar_flag = nullptr, *vbar_list = nullptr, *vbar_count = nullptr; // ... and its brick lists
+        double *bar = nullptr;   // point adjoints (28-line layout)
+        double *part = nullptr;  // G2P part of the point adjoints
+        double *forces = nullptr;
+        double *theta = nullptr; // mu_bar, kappa_bar (2 doubles after the force samples)
+        int *flags = nullptr;    // crushed / liquid points seen
+        std::vector<double> centers, times; // indenter center and time per substep
+        size_t ckpt_cap = 0, traj_cap = 0, grid_cap = 0, pt_cap = 0, f_cap = 0;
+        bool have_forward = false;
+    } adj;
+
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -301,6 +323,16 @@ void free_all(icempm_ctx *ctx) {
     }
     f(ctx->ens.d_slots);
     ctx->ens.d_slots = nullptr;
+    {
+        auto &A = ctx->adj;
+        // theta lives inside the forces allocation
+        for (double *q : {A.ckpt, A.traj, A.G, A.V, A.Gbar, A.vbar, A.bar, A.part, A.forces})
+            if (q) (void)hipFree(q);
+        for (int32_t *q : {A.vbar_flag, A.vbar_list, A.vbar_count})
+            if (q) (void)hipFree(q);
+        if (A.flags) (void)hipFree(A.flags);
+        A = icempm_ctx::Adjoint{};
+    }
     f(ctx->scratch);
     f(ctx->seg_start);
     f(ctx->seg_brick);
@@ -1005,6 +1037,248 @@ int icempm_read_timings(icempm_ctx *ctx, double *ms_out, long long *launches_out
     for (int i = 0; i < ICEMPM_N_PHASES; i++) {
         if (ms_out) ms_out[i] = ctx->ms[i];
         if (launches_out) launches_out[i] = ctx->launches[i];
+    }
+    return ICEMPM_OK;
+}
+
+extern "C++" {
+namespace {
+
+// (Re)allocates a device array of at least `count` doubles.
+int adj_reserve(double **p, size_t &cap, size_t count, const char *what) {
+    if (*p && cap >= count) return ICEMPM_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    ICEMPM_TRY(dev_alloc(p, count, what));
+    cap = count;
+    return ICEMPM_OK;
+}
+
+PtsView adj_view(double *base, int n) { return PtsView{base, (long long)n}; }
+
+// The lines G2P reads (utility, Jp_inv, position, Fe) from one state to the next.
+int adj_copy_g2p_inputs(icempm_ctx *ctx, const double *from, double *to) {
+    const int n = ctx->adj.n;
+    const int spans[3][2] = {{ICEMPM_IDX_UTILITY, 1}, {ICEMPM_IDX_JP_INV, 4}, {ICEMPM_FE00, 9}};
+    for (const auto &sp : spans)
+        ICEMPM_HIP(hipMemcpyAsync(to + (size_t)sp[0] * n, from + (size_t)sp[0] * n, sizeof(double) * n * sp[1],
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+    return ICEMPM_OK;
+}
+
+// Substeps s0 .. s0 + count - 1 of the step's own kernels from the state in
+// slot(0); the state after substep s0 + j goes to slot(j + 1). P2G of each
+// substep after the first is fused with the G2P before it (velocity and C
+// stored), on the context's two grids. With keep, the grid after P2G and
+// after the update of substep s0 + j is copied to G + 4 gp j / V + 4 gp j;
+// with record, the force samples are accumulated.
+template <typename Slot>
+int adj_run(icempm_ctx *ctx, int s0, int count, Slot slot, bool keep, bool record) {
+    auto &A = ctx->adj;
+    const long long gp = ctx->kp.grid_pitch;
+    const int S = A.run.sample_every;
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    int ga = 0, gb = 1;
+    ICEMPM_TRY(do_reset_grid(ctx, ga));
+    ICEMPM_HIP(launch_p2g(ctx->kp, adj_view(slot(0), A.n), A.n, seg, ctx->gstate(ga), ctx->grids[ga].cur, ctx->err,
+                          ctx->p2g_blocks, ctx->stream));
+    for (int j = 0; j < count; j++) {
+        const int s = s0 + j;
+        const double *c = &A.centers[3 * s];
+        if (record && s % S == 0) ICEMPM_TRY(zero_bins(ctx));
+        if (keep)
+            ICEMPM_HIP(hipMemcpyAsync(A.G + (size_t)4 * gp * j, ctx->grids[ga].grid, sizeof(double) * 4 * gp,
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+        ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(ga), ctx->grids[ga].cur, c[0], c[1], c[2], A.times[s],
+                                       ctx->bins, ctx->stream));
+        if (record && (s + 1) % S == 0)
+            ICEMPM_HIP(launch_sum_bins(ctx->bins, ctx->nbins, 1.0 / S, A.forces + 3 * (s / S), ctx->stream));
+        if (keep)
+            ICEMPM_HIP(hipMemcpyAsync(A.V + (size_t)4 * gp * j, ctx->grids[ga].grid, sizeof(double) * 4 * gp,
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+        ICEMPM_TRY(adj_copy_g2p_inputs(ctx, slot(j), slot(j + 1)));
+        const PtsView out = adj_view(slot(j + 1), A.n);
+        if (j + 1 < count) {
+            ICEMPM_TRY(do_reset_grid(ctx, gb));
+            ICEMPM_HIP(launch_g2p2g(ctx->kp, out, A.n, seg, ctx->grids[ga].grid, ctx->gstate(gb), ctx->grids[gb].cur, 0,
+                                    1, ctx->err, ctx->work, ctx->p2g_blocks, ctx->stream));
+            std::swap(ga, gb);
+        } else {
+            ICEMPM_HIP(launch_g2p(ctx->kp, out, A.n, seg, ctx->grids[ga].grid, 0, ctx->err, ctx->work,
+                                  ctx->g2p_blocks, ctx->stream));
+        }
+    }
+    return ICEMPM_OK;
+}
+
+int adj_check(icempm_ctx *ctx) {
+    int flags[2] = {0, 0}, code = 0;
+    ICEMPM_HIP(hipMemcpyAsync(flags, ctx->adj.flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipMemcpyAsync(&code, ctx->err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    if (code) return fail(ICEMPM_ERR_DEVICE_FLAG, "adjoint run: device error indicator " + std::to_string(code));
+    if (flags[1]) return fail(ICEMPM_ERR_INVALID_ARG, "adjoint run: liquid points are not differentiated");
+    if (flags[0])
+        return fail(ICEMPM_ERR_PLASTIC, "adjoint run: a point crushed; the return mapping is not differentiated");
+    return ICEMPM_OK;
+}
+
+} // namespace
+} // extern "C++"
+
+int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double *forces_out) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (!run || !forces_out) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (ctx->ens.n > 0) return fail(ICEMPM_ERR_INVALID_ARG, "the adjoint runs one scene, not an ensemble");
+    if (ctx->prms.SetupType == 2) return fail(ICEMPM_ERR_INVALID_ARG, "the adjoint does not cover SetupType 2");
+    if (ctx->nbins <= 0) return fail(ICEMPM_ERR_INVALID_ARG, "the force samples need IndenterSubdivisions > 0");
+    const icempm_AdjointRun R = *run;
+    if (R.n_steps <= 0 || R.sample_every <= 0 || R.checkpoint_every <= 0 || R.n_steps % R.sample_every ||
+        R.n_steps % R.checkpoint_every)
+        return fail(ICEMPM_ERR_INVALID_ARG, "n_steps must be a positive multiple of sample_every and checkpoint_every");
+    auto &A = ctx->adj;
+    A.have_forward = false;
+    A.run = R;
+    A.n = ctx->n;
+    A.nseg = R.n_steps / R.checkpoint_every;
+    const int K = R.checkpoint_every, n = A.n;
+    const long long gp = ctx->kp.grid_pitch;
+    const size_t st = (size_t)ICEMPM_N_PTS_ARRAYS * std::max(n, 1);
+    ICEMPM_TRY(adj_reserve(&A.ckpt, A.ckpt_cap, st * A.nseg, "adjoint checkpoints"));
+    ICEMPM_TRY(adj_reserve(&A.traj, A.traj_cap, st * (K + 1), "adjoint segment states"));
+    const size_t need_grid = (size_t)gp * 4 * (2 * K + 2);
+    if (A.grid_cap < need_grid) {
+        for (double **q : {&A.G, &A.V, &A.Gbar, &A.vbar}) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
+        for (int32_t **q : {&A.vbar_flag, &A.vbar_list, &A.vbar_count}) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
+        A.grid_cap = 0;
+        ICEMPM_TRY(dev_alloc(&A.G, (size_t)gp * 4 * K, "adjoint grids"));
+        ICEMPM_TRY(dev_alloc(&A.V, (size_t)gp * 4 * K, "adjoint grids"));
+        ICEMPM_TRY(dev_alloc(&A.Gbar, (size_t)gp * 4, "adjoint grid adjoints"));
+        ICEMPM_TRY(dev_alloc(&A.vbar, (size_t)gp * 4, "adjoint velocity adjoints"));
+        ICEMPM_TRY(dev_alloc(&A.vbar_flag, (size_t)ctx->nbricks, "adjoint brick flags"));
+        ICEMPM_TRY(dev_alloc(&A.vbar_list, (size_t)ctx->nbricks, "adjoint brick list"));
+        ICEMPM_TRY(dev_alloc(&A.vbar_count, 2, "adjoint brick count"));
+        ICEMPM_HIP(hipMemsetAsync(A.vbar_flag, 0, sizeof(int32_t) * ctx->nbricks, ctx->stream));
+        ICEMPM_HIP(hipMemsetAsync(A.vbar_count, 0, sizeof(int32_t) * 2, ctx->stream));
+        A.grid_cap = need_grid;
+    }
+    if (A.pt_cap < st) {
+        for (double **q : {&A.bar, &A.part}) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
+        A.pt_cap = 0;
+        ICEMPM_TRY(dev_alloc(&A.bar, st, "point adjoints"));
+        ICEMPM_TRY(dev_alloc(&A.part, st, "point adjoints"));
+        A.pt_cap = st;
+    }
+    const int nsamp = R.n_steps / R.sample_every;
+    ICEMPM_TRY(adj_reserve(&A.forces, A.f_cap, 3 * (size_t)nsamp + 2, "force samples"));
+    A.theta = A.forces + 3 * nsamp;
+    if (!A.flags) ICEMPM_TRY(dev_alloc(&A.flags, 2, "adjoint flags"));
+    ICEMPM_HIP(hipMemsetAsync(A.flags, 0, sizeof(int) * 2, ctx->stream));
+    ICEMPM_TRY(clear_err(ctx));
+    ICEMPM_HIP(hipMemsetAsync(A.forces, 0, sizeof(double) * 3 * nsamp, ctx->stream));
+    // the context's points, in the order the device keeps them, are the initial state
+    for (int l = 0; l < ICEMPM_N_PTS_ARRAYS && n > 0; l++)
+        ICEMPM_HIP(hipMemcpyAsync(A.traj + (size_t)l * n, ctx->pts[ctx->cur_pts] + (size_t)l * ctx->pitch,
+                                  sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    ICEMPM_HIP(launch_adj_flags(adj_view(A.traj, n), n, A.flags, ctx->stream));
+    A.centers.assign(3 * (size_t)R.n_steps, 0.0);
+    A.times.assign((size_t)R.n_steps, 0.0);
+    double t = R.t0;
+    for (int s = 0; s < R.n_steps; s++) {
+        t += ctx->prms.InitialTimeStep;
+        A.times[s] = t;
+        for (int d = 0; d < 3; d++) A.centers[3 * s + d] = ctx->ind.center_initial[d] + t * ctx->ind.velocity[d];
+    }
+    // the grids' brick lists must be consistent for the step's reset
+    for (int g = 0; g < 2; g++) {
+        ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(g), ctx->nbricks, ctx->stream));
+        ctx->grids[g].cur = 0;
+    }
+    // segment by segment, keeping each start; states ping-pong between two slots
+    double *cur = A.traj, *nxt = A.traj + st;
+    for (int seg_i = 0; seg_i < A.nseg; seg_i++) {
+        ICEMPM_HIP(hipMemcpyAsync(A.ckpt + st * seg_i, cur, sizeof(double) * st, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        double *a = cur, *b = nxt;
+        ICEMPM_TRY(adj_run(ctx, seg_i * K, K, [&](int j) { return (j & 1) ? b : a; }, false, true));
+        if (K & 1) std::swap(cur, nxt);
+    }
+    // the crushed bit is sticky: the final state shows whether any point crushed
+    ICEMPM_HIP(launch_adj_flags(adj_view(cur, n), n, A.flags, ctx->stream));
+    ICEMPM_HIP(hipMemcpyAsync(forces_out, A.forces, sizeof(double) * 3 * nsamp, hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_TRY(adj_check(ctx));
+    A.have_forward = true;
+    return ICEMPM_OK;
+}
+
+int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[2]) {
+    ICEMPM_TRY(check_ready(ctx));
+    auto &A = ctx->adj;
+    if (!A.have_forward) return fail(ICEMPM_ERR_NOT_READY, "icempm_adjoint_forward has not completed");
+    if (!dL_dforces || !grad_out) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (A.n != ctx->n) return fail(ICEMPM_ERR_NOT_READY, "the points changed since icempm_adjoint_forward");
+    const int K = A.run.checkpoint_every, n = A.n, S = A.run.sample_every;
+    const long long gp = ctx->kp.grid_pitch;
+    const size_t st = (size_t)ICEMPM_N_PTS_ARRAYS * std::max(n, 1);
+    ICEMPM_HIP(hipMemsetAsync(A.bar, 0, sizeof(double) * st, ctx->stream));
+    ICEMPM_HIP(hipMemsetAsync(A.part, 0, sizeof(double) * st, ctx->stream));
+    ICEMPM_HIP(hipMemsetAsync(A.theta, 0, sizeof(double) * 2, ctx->stream));
+    const PtsView bar = adj_view(A.bar, n), part = adj_view(A.part, n);
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    GridState gb;
+    gb.grid = A.vbar;
+    gb.brick_flag = A.vbar_flag;
+    gb.brick_list[0] = gb.brick_list[1] = A.vbar_list;
+    gb.brick_count = A.vbar_count;
+    for (int g = 0; g < 2; g++) {
+        ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(g), ctx->nbricks, ctx->stream));
+        ctx->grids[g].cur = 0;
+    }
+    for (int seg_i = A.nseg - 1; seg_i >= 0; seg_i--) {
+        const int s0 = seg_i * K;
+        ICEMPM_HIP(hipMemcpyAsync(A.traj, A.ckpt + st * seg_i, sizeof(double) * st, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        ICEMPM_TRY(adj_run(ctx, s0, K, [&](int j) { return A.traj + st * j; }, true, false));
+        for (int j = K - 1; j >= 0; j--) {
+            const int s = s0 + j;
+            double fbar[3];
+            for (int i = 0; i < 3; i++) fbar[i] = dL_dforces[3 * (s / S) + i] / S;
+            const PtsView s_0 = adj_view(A.traj + st * j, n), s_1 = adj_view(A.traj + st * (j + 1), n);
+            const double *G = A.G + (size_t)4 * gp * j, *V = A.V + (size_t)4 * gp * j;
+            ICEMPM_HIP(hipMemsetAsync(A.vbar, 0, sizeof(double) * 4 * gp, ctx->stream));
+            ICEMPM_HIP(launch_adj_g2p_bwd(ctx->kp, s_0, s_1, bar, part, n, seg, V, gb, 0, ctx->p2g_blocks,
+                                          ctx->stream));
+            ICEMPM_HIP(launch_adj_update_bwd(ctx->kp, G, A.vbar, A.Gbar, &A.centers[3 * s], fbar, ctx->stream));
+            ICEMPM_HIP(launch_adj_p2g_bwd(ctx->kp, s_0, A.Gbar, part, bar, A.theta, n, ctx->stream));
+        }
+    }
+    double th[2] = {0.0, 0.0};
+    ICEMPM_HIP(hipMemcpyAsync(th, A.theta, sizeof(th), hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_TRY(adj_check(ctx));
+    // Lame chain rule (ComputeLame, parameters_sim_3d.cpp:164-169):
+    // mu = E / (2 (1 + nu)), lambda = E nu / ((1 + nu)(1 - 2 nu)), kappa = 2 mu / 3 + lambda
+    const double E = ctx->prms.YoungsModulus, nu = ctx->prms.PoissonsRatio;
+    const double D = (1 + nu) * (1 - 2 * nu);
+    const double dmu_dE = 1.0 / (2 * (1 + nu)), dmu_dnu = -E / (2 * (1 + nu) * (1 + nu));
+    const double dl_dE = nu / D, dl_dnu = E * (1 + 2 * nu * nu) / (D * D);
+    const double dk_dE = 2.0 / 3.0 * dmu_dE + dl_dE, dk_dnu = 2.0 / 3.0 * dmu_dnu + dl_dnu;
+    grad_out[0] = th[0] * dmu_dE + th[1] * dk_dE;
+    grad_out[1] = th[0] * dmu_dnu + th[1] * dk_dnu;
+    grad_out[2] = 0.0; // friction angle and cohesion act through the return mapping only
+    grad_out[3] = 0.0;
+    if (raw_out) {
+        raw_out[0] = th[0];
+        raw_out[1] = th[1];
     }
     return ICEMPM_OK;
 }

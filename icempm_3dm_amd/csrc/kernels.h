@@ -75,4 +75,20 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
                         int cur_b, int record_pq, int store_vc, int *err, int *lists, int n_blocks,
                         hipStream_t stream);
 
+// Reverse mode of the step for the calibration adjoint (kernels_adjoint.hip);
+// grids in the step's bricked layout. V: the grid after the update (m, v);
+// gb: Vbar (v_bar in lines 1-3; lists marked, never reset); G: the grid after
+// P2G (m, mv); Gbar: (m_bar, mv_bar) over every node.
+hipError_t launch_adj_g2p_bwd(const KParams &kp, PtsView s0, PtsView s1, PtsView bar, PtsView part, int n,
+                              SegInfo seg, const double *V, GridState gb, int cur_b, int n_blocks,
+                              hipStream_t stream);
+hipError_t launch_adj_update_bwd(const KParams &kp, const double *G, const double *Vbar, double *Gbar,
+                                 const double c[3], const double fbar[3], hipStream_t stream);
+hipError_t launch_adj_p2g_bwd(const KParams &kp, PtsView s0, const double *Gbar, PtsView part, PtsView bar,
+                              double *theta, int n, hipStream_t stream);
+// out[0..2] += scale * (sum of the indenter accumulators per component).
+hipError_t launch_sum_bins(const double *bins, long long nbins, double scale, double *out, hipStream_t stream);
+// flags[0] |= 1 if a point is crushed, flags[1] |= 1 if a point is liquid.
+hipError_t launch_adj_flags(PtsView st, int n, int *flags, hipStream_t stream);
+
 } // namespace icempm

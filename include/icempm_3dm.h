@@ -39,6 +39,7 @@ extern "C" {
 #define ICEMPM_ERR_CAPACITY 4      /* host or device capacity exceeded */
 #define ICEMPM_ERR_DEVICE_FLAG 5   /* the device error indicator is set */
 #define ICEMPM_ERR_NO_DEVICE 6     /* no HIP device is visible */
+#define ICEMPM_ERR_PLASTIC 7       /* adjoint run: a point crushed (no adjoint of the return mapping) */
 
 /* ---- device error indicator values -----------------------------------
  * 70..75: a particle's 3x3x3 stencil leaves the grid (mpm_kernels.cuh:70-75:
@@ -47,6 +48,10 @@ extern "C" {
  * The indicator keeps the largest code raised since the last
  * update_constants / clear. Offending particles are skipped, not written. */
 #define ICEMPM_DEVERR_NONFINITE 76
+/* Adjoint runs only: 90 a point crushed, 91 a liquid point (no adjoint of
+ * the water pressure). */
+#define ICEMPM_ADJ_PLASTIC 90
+#define ICEMPM_ADJ_UNSUPPORTED 91
 
 /* ---- indenter law (extension) -----------------------------------------
  * The reference's indenter is an infinite cylinder along z of radius
@@ -273,6 +278,38 @@ int icempm_ensemble_transfer_points(icempm_ctx *ctx, const double *hssoa, unsign
  * grid and the offset of its IndenterSubdivisions*GridZ*3 accumulators in the
  * concatenated array. Either output may be NULL. */
 int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long long *bins_offset_out);
+
+/* Calibration adjoint (no reference counterpart: the reference has no
+ * differentiation). A run of n_steps substeps from the context's current
+ * points (which it leaves untouched), with the parameters and indenter law
+ * of the context, producing force samples F_k: the total indenter force
+ * (sum over contact nodes of (v_before - v_after) m / dt,
+ * mpm_kernels.cuh:189-201) averaged over substeps [k S, (k+1) S),
+ * S = sample_every. The indenter center at substep s is
+ * center_initial + (t0 + (s+1) dt) velocity. checkpoint_every = K: the
+ * forward keeps the point state every K substeps; the backward replays each
+ * K-substep segment from its checkpoint (keeping that segment's states and
+ * grids) and runs the reverse mode through it. n_steps must be a multiple
+ * of S and of K. Device memory: about (n_steps/K + K + 4) * 28 * 8 bytes per
+ * point plus (7 K + 7) * 8 bytes per grid node. The model differentiated is
+ * the elastic step of one scene (SetupType 0 or 1, solid points): a point
+ * that crushes ends the forward with ICEMPM_ERR_PLASTIC (the return mapping
+ * is not differentiated), liquid points with ICEMPM_ERR_DEVICE_FLAG (code
+ * ICEMPM_ADJ_UNSUPPORTED). */
+typedef struct icempm_AdjointRun {
+    int n_steps;
+    int sample_every;
+    int checkpoint_every;
+    double t0;
+} icempm_AdjointRun;
+/* Forward: writes 3 * (n_steps / sample_every) doubles to forces_out. */
+int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double *forces_out);
+/* Reverse mode of the last icempm_adjoint_forward for a loss L with
+ * dL/dF_k given (3 per sample, same layout as forces_out). grad_out[4]:
+ * dL/d(YoungsModulus, PoissonsRatio, friction angle in degrees, cohesion);
+ * the last two are exactly 0 in the elastic regime the forward guarantees.
+ * raw_out[2] (may be NULL): dL/dmu and dL/dkappa. */
+int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[2]);
 
 /* Synthetic input (no reference counterpart; replaces the HDF5 point cloud of
  * SnapshotManager::LoadRawPoints for generated scenes): a block of

@@ -757,3 +757,387 @@ int oracle_step(const icempm_SimParams3D *prms, const icempm_Indenter *ind, doub
 }
 
 } // extern "C"
+
+// ====================================================================
+// Calibration adjoint (no reference counterpart: the reference has no
+// differentiation). Loss L = sum_k <dL/dF_k, F_k> to first order, where F_k
+// is the total indenter force averaged over the substeps of sample k
+// (sample_every substeps each; the per-substep force is the sum over
+// contact nodes of (v_before - v_after) m / dt, mpm_kernels.cuh:189-201).
+// Reverse mode over the elastic step: P2G (Kirchhoff stress, APIC), the
+// grid update (gravity, velocity clamp, indenter contact, walls) and G2P.
+// A point that crushes ends the run with ICEMPM_ADJ_PLASTIC: the plastic
+// return mapping is not differentiated here.
+// ====================================================================
+namespace {
+
+constexpr int kAdjPlastic = 90;
+
+inline void bspline_dw(double f, double dw[3]) {
+    dw[0] = -(1.5 - f);
+    dw[1] = -2.0 * (f - 1.0);
+    dw[2] = f - 0.5;
+}
+
+struct PtState {
+    double x[3], v[3];
+    Mat3 F, C;
+};
+
+inline PtState load_state(const double *soa, int pitch, int i) {
+    PtState s;
+    for (int a = 0; a < 3; a++) {
+        s.x[a] = soa[i + (size_t)pitch * (ICEMPM_POSX + a)];
+        s.v[a] = soa[i + (size_t)pitch * (ICEMPM_VELX + a)];
+    }
+    for (int k = 0; k < 9; k++) {
+        s.F.a[k] = soa[i + (size_t)pitch * (ICEMPM_FE00 + k)];
+        s.C.a[k] = soa[i + (size_t)pitch * (ICEMPM_BP00 + k)];
+    }
+    return s;
+}
+
+// Forward grid update of one node (the oracle_update_nodes branch chain),
+// keeping what the reverse pass needs. Returns the node's contact force.
+struct NodeFwd {
+    double u0[3], u1[3], u2[3], u3[3], v[3];
+    bool clamp, contact;
+    double n[3];
+    bool wall_zero[3];
+};
+
+void node_forward(const icempm_SimParams3D &P, const icempm_Indenter &ind, const double center[3], int gi[3],
+                  double mass, const double mv[3], NodeFwd &nf, double force[3]) {
+    const double h = P.cellsize, dt = P.InitialTimeStep;
+    const bool sphere = ind.shape == ICEMPM_INDENTER_SPHERE;
+    for (int i = 0; i < 3; i++) nf.u0[i] = mv[i] / mass;
+    for (int i = 0; i < 3; i++) nf.u1[i] = nf.u0[i];
+    nf.u1[1] -= P.dt_Gravity;
+    double vsq = nf.u1[0] * nf.u1[0] + nf.u1[1] * nf.u1[1] + nf.u1[2] * nf.u1[2];
+    nf.clamp = vsq > P.vmax_squared;
+    for (int i = 0; i < 3; i++) nf.u2[i] = nf.clamp ? nf.u1[i] / std::sqrt(vsq) * P.vmax : nf.u1[i];
+    for (int i = 0; i < 3; i++) nf.u3[i] = nf.u2[i];
+    nf.contact = false;
+    force[0] = force[1] = force[2] = 0.0;
+    if (P.SetupType != 2) {
+        double g[3] = {gi[0] * h, gi[1] * h, gi[2] * h};
+        double n[3] = {g[0] - center[0], g[1] - center[1], sphere ? g[2] - center[2] : 0.0};
+        double nsq = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        if (nsq < P.IndRSq) {
+            double vrel[3] = {nf.u2[0] - ind.velocity[0], nf.u2[1] - ind.velocity[1], nf.u2[2] - ind.velocity[2]};
+            if (nsq > 0) {
+                double nrm = std::sqrt(nsq);
+                for (int i = 0; i < 3; i++) n[i] /= nrm;
+            }
+            double vn = vrel[0] * n[0] + vrel[1] * n[1] + vrel[2] * n[2];
+            if (vn < 0) {
+                nf.contact = true;
+                for (int i = 0; i < 3; i++) {
+                    nf.n[i] = n[i];
+                    nf.u3[i] = ind.velocity[i] + (vrel[i] - n[i] * vn);
+                    force[i] = (nf.u2[i] - nf.u3[i]) * mass / dt;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < 3; i++) {
+        nf.v[i] = nf.u3[i];
+        nf.wall_zero[i] = false;
+    }
+    const int GX = P.GridXTotal, GY = P.GridY, GZ = P.GridZ;
+    if (P.SetupType != 2) {
+        if (gi[1] <= 2) nf.wall_zero[0] = nf.wall_zero[1] = nf.wall_zero[2] = true;
+    } else if (gi[1] <= 2 && nf.v[1] < 0) {
+        nf.wall_zero[1] = true;
+    }
+    if (gi[1] >= GY - 3 && nf.v[1] > 0) nf.wall_zero[1] = true;
+    if (gi[0] <= 2 && nf.v[0] < 0) nf.wall_zero[0] = true;
+    else if (gi[0] >= GX - 3 && nf.v[0] > 0) nf.wall_zero[0] = true;
+    if (gi[2] <= 2 && nf.v[2] < 0) nf.wall_zero[2] = true;
+    else if (gi[2] >= GZ - 3 && nf.v[2] > 0) nf.wall_zero[2] = true;
+    for (int i = 0; i < 3; i++)
+        if (nf.wall_zero[i]) nf.v[i] = 0.0;
+}
+
+// d(tau):tau_bar for the Kirchhoff stress tau = mu J^(-2/3) dev(F F^T) +
+// (kappa/2)(J^2 - 1) I: returns F_bar (added), mu_bar and kappa_bar (added).
+void kirchhoff_adjoint(double mu, double kappa, const Mat3 &F, const Mat3 &tb, Mat3 &Fbar, double &mub,
+                       double &kappab) {
+    const double J = det3(F);
+    const Mat3 b = mul_abt(F, F);
+    const Mat3 db = dev(b);
+    const double s = rcbrt(J * J); // J^(-2/3)
+    double dbt = 0, trt = tb(0, 0) + tb(1, 1) + tb(2, 2);
+    for (int k = 0; k < 9; k++) dbt += db.a[k] * tb.a[k];
+    mub += s * dbt;
+    kappab += 0.5 * (J * J - 1.0) * trt;
+    // cofactor matrix: J F^{-T}
+    Mat3 cof;
+    cof(0, 0) = F(1, 1) * F(2, 2) - F(1, 2) * F(2, 1);
+    cof(0, 1) = F(1, 2) * F(2, 0) - F(1, 0) * F(2, 2);
+    cof(0, 2) = F(1, 0) * F(2, 1) - F(1, 1) * F(2, 0);
+    cof(1, 0) = F(0, 2) * F(2, 1) - F(0, 1) * F(2, 2);
+    cof(1, 1) = F(0, 0) * F(2, 2) - F(0, 2) * F(2, 0);
+    cof(1, 2) = F(0, 1) * F(2, 0) - F(0, 0) * F(2, 1);
+    cof(2, 0) = F(0, 1) * F(1, 2) - F(0, 2) * F(1, 1);
+    cof(2, 1) = F(0, 2) * F(1, 0) - F(0, 0) * F(1, 2);
+    cof(2, 2) = F(0, 0) * F(1, 1) - F(0, 1) * F(1, 0);
+    // dev(tb) symmetrised, times F
+    Mat3 T = dev(tb);
+    Mat3 TS;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) TS(r, c) = T(r, c) + T(c, r);
+    const Mat3 TSF = mul(TS, F);
+    // dJ = cof : dF; d(J^(-2/3)) = -(2/3) J^(-2/3) dJ / J
+    const double cJ = mu * s * (-2.0 / 3.0) * dbt / J + kappa * J * trt;
+    for (int k = 0; k < 9; k++) Fbar.a[k] += mu * s * TSF.a[k] + cJ * cof.a[k];
+}
+
+} // namespace
+
+extern "C" {
+
+int oracle_adj_forward(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
+                       int n, int n_steps, int sample_every, double *forces) {
+    const icempm_SimParams3D &P = *prms;
+    const size_t nn = (size_t)P.GridXTotal * P.GridY * P.GridZ;
+    std::vector<double> soa(soa0, soa0 + (size_t)pitch * ICEMPM_N_PTS_ARRAYS);
+    std::vector<double> grid(nn * 4);
+    const int nsamp = n_steps / sample_every;
+    for (int k = 0; k < 3 * nsamp; k++) forces[k] = 0.0;
+    double t = 0.0;
+    for (int s = 0; s < n_steps; s++) {
+        t += P.InitialTimeStep;
+        double center[3];
+        for (int d = 0; d < 3; d++) center[d] = ind->center_initial[d] + t * ind->velocity[d];
+        std::fill(grid.begin(), grid.end(), 0.0);
+        int code = oracle_p2g(prms, soa.data(), pitch, n, grid.data(), 1);
+        if (code) return code;
+        double fs[3] = {0, 0, 0};
+        for (size_t idx = 0; idx < nn; idx++) {
+            const double mass = grid[idx];
+            if (mass == 0) continue;
+            int gi[3] = {(int)(idx / ((size_t)P.GridZ * P.GridY)), (int)((idx / P.GridZ) % P.GridY),
+                         (int)(idx % P.GridZ)};
+            double mv[3] = {grid[nn + idx], grid[2 * nn + idx], grid[3 * nn + idx]};
+            NodeFwd nf;
+            double f[3];
+            node_forward(P, *ind, center, gi, mass, mv, nf, f);
+            for (int i = 0; i < 3; i++) {
+                fs[i] += f[i];
+                grid[(1 + i) * nn + idx] = nf.v[i];
+            }
+        }
+        if (s / sample_every < nsamp)
+            for (int i = 0; i < 3; i++) forces[3 * (s / sample_every) + i] += fs[i] / sample_every;
+        code = oracle_g2p(prms, soa.data(), pitch, n, grid.data(), 0, 1);
+        if (code) return code;
+        for (int i = 0; i < n; i++)
+            if (util_bits(soa[i]) & ICEMPM_STATUS_CRUSHED) return kAdjPlastic;
+    }
+    return 0;
+}
+
+int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
+                        int n, int n_steps, int sample_every, const double *dL_dF, double *grad_mu_kappa) {
+    const icempm_SimParams3D &P = *prms;
+    const size_t nn = (size_t)P.GridXTotal * P.GridY * P.GridZ;
+    const size_t lines = (size_t)pitch * ICEMPM_N_PTS_ARRAYS;
+    const double h = P.cellsize, h_inv = P.cellsize_inv, dt = P.InitialTimeStep;
+    const double mu = P.mu, kappa = P.kappa, D = P.dt_vol_Dpinv;
+    const int nsamp = n_steps / sample_every;
+    // forward, keeping every substep's start state
+    std::vector<double> states((size_t)(n_steps + 1) * lines);
+    std::copy(soa0, soa0 + lines, states.begin());
+    std::vector<double> grid(nn * 4);
+    std::vector<double> centers(3 * (size_t)n_steps);
+    double t = 0.0;
+    for (int s = 0; s < n_steps; s++) {
+        double *cur = &states[(size_t)(s + 1) * lines];
+        std::copy(&states[(size_t)s * lines], &states[(size_t)s * lines] + lines, cur);
+        t += dt;
+        for (int d = 0; d < 3; d++) centers[3 * s + d] = ind->center_initial[d] + t * ind->velocity[d];
+        std::fill(grid.begin(), grid.end(), 0.0);
+        int code = oracle_p2g(prms, cur, pitch, n, grid.data(), 1);
+        if (code) return code;
+        std::vector<double> bins(std::max(P.IndenterSubdivisions * P.GridZ * 3, 1));
+        oracle_update_nodes(prms, ind, &centers[3 * s], grid.data(), bins.data(), 1);
+        code = oracle_g2p(prms, cur, pitch, n, grid.data(), 0, 1);
+        if (code) return code;
+        for (int i = 0; i < n; i++)
+            if (util_bits(cur[i]) & ICEMPM_STATUS_CRUSHED) return kAdjPlastic;
+    }
+    // reverse
+    std::vector<PtState> bar(n);
+    for (auto &b : bar) {
+        for (int a = 0; a < 3; a++) b.x[a] = b.v[a] = 0.0;
+        b.F = zero3();
+        b.C = zero3();
+    }
+    std::vector<double> gm(nn * 4), vgrid(nn * 3), vbar(nn * 3), gbar(nn * 4);
+    double mub = 0.0, kappab = 0.0;
+    for (int s = n_steps - 1; s >= 0; s--) {
+        const double *st = &states[(size_t)s * lines];
+        const double *nx = &states[(size_t)(s + 1) * lines];
+        const double *center = &centers[3 * s];
+        double fbar[3] = {0, 0, 0};
+        if (s / sample_every < nsamp)
+            for (int i = 0; i < 3; i++) fbar[i] = dL_dF[3 * (s / sample_every) + i] / sample_every;
+        // recompute the grid of substep s
+        std::fill(gm.begin(), gm.end(), 0.0);
+        oracle_p2g(prms, st, pitch, n, gm.data(), 1);
+        std::vector<NodeFwd> nodes(nn);
+        for (size_t idx = 0; idx < nn; idx++) {
+            for (int i = 0; i < 3; i++) vgrid[i * nn + idx] = 0.0;
+            if (gm[idx] == 0) continue;
+            int gi[3] = {(int)(idx / ((size_t)P.GridZ * P.GridY)), (int)((idx / P.GridZ) % P.GridY),
+                         (int)(idx % P.GridZ)};
+            double mv[3] = {gm[nn + idx], gm[2 * nn + idx], gm[3 * nn + idx]};
+            double f[3];
+            node_forward(P, *ind, center, gi, gm[idx], mv, nodes[idx], f);
+            for (int i = 0; i < 3; i++) vgrid[i * nn + idx] = nodes[idx].v[i];
+        }
+        // G2P reverse: scatter v_bar to the nodes, partial point adjoints
+        std::fill(vbar.begin(), vbar.end(), 0.0);
+        std::vector<PtState> part(n);
+        for (int p = 0; p < n; p++) {
+            const PtState S0 = load_state(st, pitch, p);
+            const PtState S1 = load_state(nx, pitch, p);
+            PtState &B = bar[p];
+            Weights W;
+            weights_and_check(P, S0.x, W);
+            Mat3 A = eye3();
+            for (int k = 0; k < 9; k++) A.a[k] += dt * S1.C.a[k];
+            Mat3 chat = B.C;
+            const Mat3 FbFt = mul_abt(B.F, S0.F);
+            for (int k = 0; k < 9; k++) chat.a[k] += dt * FbFt.a[k];
+            double vhat[3];
+            for (int a = 0; a < 3; a++) vhat[a] = B.v[a] + dt * B.x[a];
+            PtState &Q = part[p];
+            // F_bar through F' = A F
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++)
+                    Q.F(r, c) = A(0, r) * B.F(0, c) + A(1, r) * B.F(1, c) + A(2, r) * B.F(2, c);
+            double dw[3][3];
+            for (int d = 0; d < 3; d++) bspline_dw(W.fx[d], dw[d]);
+            double fb[3] = {0, 0, 0};
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    for (int c = 0; c < 3; c++) {
+                        const int o[3] = {a, b, c};
+                        const double Wa = W.w[a][0] * W.w[b][1] * W.w[c][2];
+                        const double dWa[3] = {dw[0][a] * W.w[b][1] * W.w[c][2], W.w[a][0] * dw[1][b] * W.w[c][2],
+                                               W.w[a][0] * W.w[b][1] * dw[2][c]};
+                        const size_t idx = node_index(P, W.base[0] + a, W.base[1] + b, W.base[2] + c);
+                        double va[3], df[3];
+                        for (int i = 0; i < 3; i++) {
+                            va[i] = vgrid[i * nn + idx];
+                            df[i] = o[i] - W.fx[i];
+                        }
+                        double cd[3]; // chat (a - f)
+                        for (int r = 0; r < 3; r++) cd[r] = chat(r, 0) * df[0] + chat(r, 1) * df[1] + chat(r, 2) * df[2];
+                        for (int r = 0; r < 3; r++) vbar[r * nn + idx] += Wa * (vhat[r] + 4.0 * h_inv * cd[r]);
+                        const double g = va[0] * vhat[0] + va[1] * vhat[1] + va[2] * vhat[2] +
+                                         4.0 * h_inv * (va[0] * cd[0] + va[1] * cd[1] + va[2] * cd[2]);
+                        for (int d = 0; d < 3; d++) {
+                            const double ctv = chat(0, d) * va[0] + chat(1, d) * va[1] + chat(2, d) * va[2];
+                            fb[d] += dWa[d] * g - 4.0 * h_inv * Wa * ctv;
+                        }
+                    }
+            for (int d = 0; d < 3; d++) Q.x[d] = B.x[d] + fb[d] * h_inv;
+        }
+        // grid update reverse
+        std::fill(gbar.begin(), gbar.end(), 0.0);
+        for (size_t idx = 0; idx < nn; idx++) {
+            const double mass = gm[idx];
+            if (mass == 0) continue;
+            const NodeFwd &nf = nodes[idx];
+            double u3b[3], u2b[3], u1b[3];
+            for (int i = 0; i < 3; i++) u3b[i] = nf.wall_zero[i] ? 0.0 : vbar[i * nn + idx];
+            double mb = 0.0;
+            if (nf.contact) {
+                // force = (u2 - u3) m / dt
+                for (int i = 0; i < 3; i++) {
+                    u3b[i] -= fbar[i] * mass / dt;
+                    mb += fbar[i] * (nf.u2[i] - nf.u3[i]) / dt;
+                }
+                const double nu = nf.n[0] * u3b[0] + nf.n[1] * u3b[1] + nf.n[2] * u3b[2];
+                for (int i = 0; i < 3; i++) u2b[i] = u3b[i] - nf.n[i] * nu + fbar[i] * mass / dt;
+            } else {
+                for (int i = 0; i < 3; i++) u2b[i] = u3b[i];
+            }
+            if (nf.clamp) {
+                const double nrm = std::sqrt(nf.u1[0] * nf.u1[0] + nf.u1[1] * nf.u1[1] + nf.u1[2] * nf.u1[2]);
+                double uh[3] = {nf.u1[0] / nrm, nf.u1[1] / nrm, nf.u1[2] / nrm};
+                const double d = uh[0] * u2b[0] + uh[1] * u2b[1] + uh[2] * u2b[2];
+                for (int i = 0; i < 3; i++) u1b[i] = P.vmax / nrm * (u2b[i] - uh[i] * d);
+            } else {
+                for (int i = 0; i < 3; i++) u1b[i] = u2b[i];
+            }
+            // u0 = mv / m
+            double dot = 0;
+            for (int i = 0; i < 3; i++) {
+                gbar[(1 + i) * nn + idx] = u1b[i] / mass;
+                dot += u1b[i] * nf.u0[i];
+            }
+            gbar[idx] = mb - dot / mass;
+        }
+        // P2G reverse
+        for (int p = 0; p < n; p++) {
+            const PtState S0 = load_state(st, pitch, p);
+            const long long util = util_bits(st[p]);
+            const double m = (util & ICEMPM_STATUS_LIQUID) ? P.LiquidParticleMass : P.ParticleMass;
+            Weights W;
+            weights_and_check(P, S0.x, W);
+            const Mat3 tau = kirchhoff(mu, kappa, S0.F);
+            Mat3 sub;
+            for (int k = 0; k < 9; k++) sub.a[k] = m * S0.C.a[k] - D * tau.a[k];
+            double dw[3][3];
+            for (int d = 0; d < 3; d++) bspline_dw(W.fx[d], dw[d]);
+            double vb[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
+            Mat3 subb = zero3();
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    for (int c = 0; c < 3; c++) {
+                        const int o[3] = {a, b, c};
+                        const double Wa = W.w[a][0] * W.w[b][1] * W.w[c][2];
+                        const double dWa[3] = {dw[0][a] * W.w[b][1] * W.w[c][2], W.w[a][0] * dw[1][b] * W.w[c][2],
+                                               W.w[a][0] * W.w[b][1] * dw[2][c]};
+                        const size_t idx = node_index(P, W.base[0] + a, W.base[1] + b, W.base[2] + c);
+                        const double mb = gbar[idx];
+                        double pb[3], dp[3];
+                        for (int i = 0; i < 3; i++) {
+                            pb[i] = gbar[(1 + i) * nn + idx];
+                            dp[i] = (o[i] - W.fx[i]) * h;
+                        }
+                        double q = m * mb;
+                        for (int r = 0; r < 3; r++) {
+                            const double sd = sub(r, 0) * dp[0] + sub(r, 1) * dp[1] + sub(r, 2) * dp[2];
+                            q += pb[r] * (S0.v[r] * m + sd);
+                            vb[r] += Wa * m * pb[r];
+                            for (int cc = 0; cc < 3; cc++) subb(r, cc) += Wa * pb[r] * dp[cc];
+                        }
+                        for (int d = 0; d < 3; d++) {
+                            const double stp = sub(0, d) * pb[0] + sub(1, d) * pb[1] + sub(2, d) * pb[2];
+                            fb[d] += dWa[d] * q - h * Wa * stp;
+                        }
+                    }
+            PtState &B = bar[p];
+            const PtState &Q = part[p];
+            for (int d = 0; d < 3; d++) {
+                B.x[d] = Q.x[d] + fb[d] * h_inv;
+                B.v[d] = vb[d];
+            }
+            for (int k = 0; k < 9; k++) B.C.a[k] = m * subb.a[k];
+            Mat3 tb;
+            for (int k = 0; k < 9; k++) tb.a[k] = -D * subb.a[k];
+            B.F = Q.F;
+            kirchhoff_adjoint(mu, kappa, S0.F, tb, B.F, mub, kappab);
+        }
+    }
+    grad_mu_kappa[0] = mub;
+    grad_mu_kappa[1] = kappab;
+    return 0;
+}
+
+} // extern "C"

@@ -38,6 +38,8 @@ _PROTOS = {
     "oracle_update_nodes": (None, [_pp, _indp, _dp, _dp, _dp, C.c_int]),
     "oracle_g2p": (C.c_int, [_pp, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int]),
     "oracle_step": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, C.c_int]),
+    "oracle_adj_forward": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    "oracle_adj_gradient": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]),
 }
 
 ORACLE_LIB_NOFMA = ROOT / "oracle" / "lib" / "libmpm_oracle_nofma.so"
@@ -160,3 +162,21 @@ def reference_indenter(p: SimParams3D) -> Indenter:
     """The reference law: z-cylinder from (indenter_x_initial, indenter_y_initial), +x at IndVelocity."""
     return Indenter.make(capi.INDENTER_CYLINDER_Z, (p.indenter_x_initial, p.indenter_y_initial, 0.0),
                          (p.IndVelocity, 0.0, 0.0))
+
+
+def adj_forward(p: SimParams3D, ind: Indenter, soa: np.ndarray, pitch: int, n: int, n_steps: int,
+                sample_every: int):
+    """Force samples of an n_steps run from soa (left untouched): (code, (n_samples, 3))."""
+    f = np.zeros((n_steps // sample_every, 3))
+    code = lib().oracle_adj_forward(C.byref(p), C.byref(ind), _d(soa), pitch, n, n_steps, sample_every, _d(f))
+    return code, f
+
+
+def adj_gradient(p: SimParams3D, ind: Indenter, soa: np.ndarray, pitch: int, n: int, n_steps: int,
+                 sample_every: int, dL_dF: np.ndarray):
+    """Reverse mode of adj_forward's run: (code, dL/dmu, dL/dkappa)."""
+    g = np.zeros(2)
+    w = np.ascontiguousarray(dL_dF, dtype=np.float64)
+    code = lib().oracle_adj_gradient(C.byref(p), C.byref(ind), _d(soa), pitch, n, n_steps, sample_every, _d(w),
+                                     _d(g))
+    return code, g[0], g[1]

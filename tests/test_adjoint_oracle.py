@@ -1,0 +1,112 @@
+"""The oracle's calibration adjoint (oracle_adj_forward / oracle_adj_gradient),
+pinned without the device.
+
+Pinned by: central finite differences of the oracle's own forward run
+(relative step 1e-6) for dL/dmu and dL/dkappa, and for dL/dE and dL/dnu
+through the Lame relations mu = E / (2 (1 + nu)), kappa = 2 mu / 3 +
+E nu / ((1 + nu)(1 - 2 nu)) (parameters_sim_3d.cpp:164-169); the forward's
+force samples against the indenter accumulators of oracle_step on the same
+run (the adjoint's forward is the checked step, not a second model); and the
+elastic-regime guard (a run with the reference's crush strengths crushes and
+is refused).
+"""
+from __future__ import annotations
+
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from icempm_3dm_amd import capi, scenes
+from icempm_3dm_amd.calibration import apply_theta, elastic_params
+
+TINY = replace(scenes.BLOCK_100K, name="adj_tiny", grid=(24, 20, 20), cells=(6, 5, 6), dt=2e-6, indenter_radius=0.06)
+N_STEPS, S = 40, 10
+
+
+def _setup(theta=None):
+    p = elastic_params(TINY, theta)
+    ind = capi.Indenter.make(*TINY.indenter())
+    hs = scenes.make_points(TINY)
+    return p, ind, hs
+
+
+def _loss(p, ind, hs, w):
+    code, F = orc.adj_forward(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S)
+    assert code == 0
+    return float((w * F).sum())
+
+
+def lame_chain(E, nu):
+    """d(mu, kappa)/d(E, nu) as a 2x2 matrix [[dmu/dE, dmu/dnu], [dkappa/dE, dkappa/dnu]]."""
+    D = (1 + nu) * (1 - 2 * nu)
+    dmu = (1 / (2 * (1 + nu)), -E / (2 * (1 + nu) ** 2))
+    dl = (nu / D, E * (1 + 2 * nu * nu) / D ** 2)
+    return np.array([dmu, (2 / 3 * dmu[0] + dl[0], 2 / 3 * dmu[1] + dl[1])])
+
+
+def test_forward_forces_match_step_accumulators():
+    p, ind, hs = _setup()
+    code, F = orc.adj_forward(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S)
+    assert code == 0
+    soa = hs.copy()
+    t = 0.0
+    for k in range(N_STEPS // S):
+        bins = np.zeros(p.IndenterArraySize())
+        code, t = orc.step(p, ind, soa.host_buffer, soa.capacity, soa.size, orc.grid_zeros(p), bins, S, t, 1)
+        assert code == 0
+        want = bins.reshape(-1, 3).sum(0) / S
+        assert np.allclose(F[k], want, rtol=1e-12, atol=1e-12 * np.abs(want).max()), (k, F[k], want)
+    assert np.abs(F[:, 1]).min() > 0  # the indenter is in contact from the first sample on
+
+
+@pytest.mark.parametrize("name,idx", [("mu", 0), ("kappa", 1)])
+def test_lame_gradient_matches_central_differences(name, idx):
+    p, ind, hs = _setup()
+    w = np.random.default_rng(3).standard_normal((N_STEPS // S, 3))
+    code, gmu, gk = orc.adj_gradient(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S, w)
+    assert code == 0
+    g = (gmu, gk)[idx]
+    base = getattr(p, name)
+    d = base * 1e-6
+    vals = []
+    for sgn in (1, -1):
+        q = p.copy()
+        setattr(q, name, base + sgn * d)
+        vals.append(_loss(q, ind, hs, w))
+    fd = (vals[0] - vals[1]) / (2 * d)
+    assert abs(g - fd) <= 1e-6 * abs(fd), (name, g, fd)
+
+
+def test_young_and_poisson_gradient_through_lame():
+    theta = (8e9, 0.3, 55.0, 1.5e6)
+    p, ind, hs = _setup(theta)
+    w = np.random.default_rng(4).standard_normal((N_STEPS // S, 3))
+    code, gmu, gk = orc.adj_gradient(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S, w)
+    assert code == 0
+    g = np.array([gmu, gk]) @ lame_chain(theta[0], theta[1])
+    for j in range(2):
+        d = theta[j] * 1e-6
+        vals = []
+        for sgn in (1, -1):
+            th = list(theta)
+            th[j] += sgn * d
+            q = p.copy()
+            apply_theta(q, th)
+            vals.append(_loss(q, ind, hs, w))
+        fd = (vals[0] - vals[1]) / (2 * d)
+        assert abs(g[j] - fd) <= 1e-6 * abs(fd), (j, g[j], fd)
+    # friction angle and cohesion do not act on an elastic run
+    for th in ((8e9, 0.3, 40.0, 1.5e6), (8e9, 0.3, 55.0, 3e6)):
+        q = p.copy()
+        apply_theta(q, th)
+        assert _loss(q, ind, hs, w) == _loss(p, ind, hs, w)
+
+
+def test_crushing_run_is_refused():
+    p = scenes.make_params(replace(TINY, indenter_speed=20.0))
+    ind = capi.Indenter.make(*replace(TINY, indenter_speed=20.0).indenter())
+    hs = scenes.make_points(TINY)
+    code, _ = orc.adj_forward(p, ind, hs.host_buffer, hs.capacity, hs.size, 200, 10)
+    assert code == 90

@@ -1,0 +1,115 @@
+"""The calibration adjoint on the device (icempm_adjoint_forward / _backward).
+
+Checked against: the oracle's forward force samples and its reverse mode
+(dL/dmu, dL/dkappa; the oracle adjoint is pinned by finite differences in
+test_adjoint_oracle.py); central finite differences of the device's own
+forward (relative step 1e-6) for dL/dE and dL/dnu; independence of the
+checkpoint period; the elastic-regime guard; and the torch.autograd
+wrapper and calibration loop.
+"""
+from __future__ import annotations
+
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from icempm_3dm_amd import capi, scenes
+from icempm_3dm_amd.calibration import (ForceCurveModel, IndenterForceCurve, TARGET_THETA, calibrate,
+                                        elastic_params)
+from parity_util import normwise
+
+pytestmark = pytest.mark.gpu
+
+TINY = replace(scenes.BLOCK_100K, name="adj_tiny", grid=(24, 20, 20), cells=(6, 5, 6), dt=2e-6, indenter_radius=0.06)
+SMALL = replace(scenes.BLOCK_100K, name="adj_small", grid=(48, 32, 32), cells=(16, 10, 12), dt=2e-6,
+                indenter_radius=0.1)
+
+
+def _oracle(scene, theta, n_steps, S, w):
+    p = elastic_params(scene, theta)
+    ind = capi.Indenter.make(*scene.indenter())
+    hs = scenes.make_points(scene)
+    code, F = orc.adj_forward(p, ind, hs.host_buffer, hs.capacity, hs.size, n_steps, S)
+    assert code == 0
+    code, gmu, gk = orc.adj_gradient(p, ind, hs.host_buffer, hs.capacity, hs.size, n_steps, S, w)
+    assert code == 0
+    return F, np.array([gmu, gk])
+
+
+@pytest.mark.parametrize("scene,n_steps,K", [(TINY, 40, 10), (SMALL, 60, 20)])
+def test_forward_and_gradient_match_oracle(require_gpu, scene, n_steps, K):
+    theta = TARGET_THETA
+    S = 10
+    w = np.random.default_rng(5).standard_normal((n_steps // S, 3))
+    F_ref, g_ref = _oracle(scene, theta, n_steps, S, w)
+    m = ForceCurveModel(scene, n_steps, S, K, theta=theta)
+    F = m.forward()
+    assert normwise(F, F_ref) <= 1e-10, (F, F_ref)
+    g, raw = m.backward(w, raw=True)
+    assert normwise(raw, g_ref) <= 1e-8, (raw, g_ref)
+    assert g[2] == 0.0 and g[3] == 0.0
+    print(scene.name, "forces", F[-1], "dL/dmu,kappa", raw, "oracle", g_ref)
+    m.close()
+
+
+def test_gradient_matches_device_finite_differences(require_gpu):
+    theta = np.array(TARGET_THETA)
+    n_steps, S, K = 60, 10, 20
+    m = ForceCurveModel(SMALL, n_steps, S, K, theta=theta)
+    w = np.random.default_rng(6).standard_normal((n_steps // S, 3))
+    m.forward(theta)
+    g = m.backward(w)
+    for j in range(2):
+        d = theta[j] * 1e-6
+        vals = []
+        for sgn in (1, -1):
+            th = theta.copy()
+            th[j] += sgn * d
+            vals.append(float((w * m.forward(th)).sum()))
+        fd = (vals[0] - vals[1]) / (2 * d)
+        assert abs(g[j] - fd) <= 1e-6 * abs(fd), (j, g[j], fd)
+    m.close()
+
+
+def test_checkpoint_period_does_not_change_the_gradient(require_gpu):
+    n_steps, S = 40, 10
+    w = np.random.default_rng(7).standard_normal((n_steps // S, 3))
+    grads = []
+    for K in (5, 10, 40):
+        m = ForceCurveModel(TINY, n_steps, S, K, theta=TARGET_THETA)
+        m.forward()
+        grads.append(m.backward(w))
+        m.close()
+    for g in grads[1:]:
+        assert normwise(g, grads[0]) <= 1e-12, grads
+
+
+def test_crushing_run_is_refused(require_gpu):
+    sc = replace(TINY, indenter_speed=20.0)
+    m = ForceCurveModel(sc, 200, 10, 20, strength_factor=1.0)
+    with pytest.raises(capi.IcempmError) as ei:
+        m.forward()
+    assert ei.value.code == capi.ERR_PLASTIC
+    with pytest.raises(capi.IcempmError):
+        m.backward(np.zeros((20, 3)))
+    m.close()
+
+
+def test_torch_function_and_calibration_loop(require_gpu):
+    torch = pytest.importorskip("torch")
+    n_steps, S, K = 40, 10, 20
+    m = ForceCurveModel(TINY, n_steps, S, K, theta=TARGET_THETA)
+    target = m.forward(TARGET_THETA)
+    theta0 = np.array(TARGET_THETA) * np.array([1.2, 0.9, 1.0, 1.0])
+    th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    F = IndenterForceCurve.apply(th, m)
+    loss = ((F - torch.from_numpy(target)) ** 2).sum()
+    loss.backward()
+    g_direct = m.backward(2 * (F.detach().numpy() - target))
+    assert np.allclose(th.grad.numpy(), g_direct, rtol=1e-12, atol=0)
+    hist = calibrate(m, target, theta0, iters=12, lr=0.03)
+    assert hist[-1][0] < 0.2 * hist[0][0], [h[0] for h in hist]
+    assert abs(hist[-1][1][0] / TARGET_THETA[0] - 1) < abs(theta0[0] / TARGET_THETA[0] - 1)
+    m.close()
