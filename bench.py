@@ -10,6 +10,12 @@ material and indenter but no time step; the headline uses dt = 2e-6 s
 block blows apart, every point crushes and spreads over ~4x more grid
 bricks) is timed after it and reported as "unstable_dt".
 
+The calibration adjoint (reverse mode of the elastic step on the
+200,000-point calibration scene: 40x25x25 cells on a 128x64x64 grid, 400
+substeps, force sampled every 10, checkpoints every 20) is timed after it and
+reported as "adjoint": the backward pass per substep (replay from the
+checkpoints included) over a plain forward substep of the same scene.
+
 The batched ensemble (64 independent scenes of a material sweep, counts
 log-uniform in [20k, 500k], each on its own 128x64x64 sub-grid, advanced in
 the same launches) is timed last and reported as "ensemble", with its
@@ -162,6 +168,41 @@ def run_ensemble(args) -> dict:
             "value": n * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps, "device_error": code}
 
 
+def run_adjoint(args) -> dict:
+    """Forward with checkpoints and backward of the calibration run, and the
+    plain step on the same scene; times per substep."""
+    from icempm_3dm_amd.calibration import CALIBRATION_SCENE, TARGET_THETA, ForceCurveModel
+
+    n_steps, S, K = 400, 10, 20
+    m = ForceCurveModel(CALIBRATION_SCENE, n_steps, S, K, theta=TARGET_THETA)
+    w = np.ones((n_steps // S, 3))
+    m.forward()
+    m.backward(w)
+    t0 = time.perf_counter()
+    F = m.forward()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = m.backward(w)
+    tb = time.perf_counter() - t0
+    gpu = m.gpu
+    gpu.set_sort_period(args.sort_period)
+    gpu.set_fusion(args.fuse != 0)
+    t = gpu.step(50, 0.0, 0)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    gpu.step(n_steps, t, 50)
+    gpu.synchronize()
+    ts = time.perf_counter() - t0
+    n = m.gpu.hssoa.size
+    m.close()
+    return {"workload": f"{CALIBRATION_SCENE.name}: {n} points on a 128x64x64 grid, {n_steps} substeps, dt=2e-6, "
+                        f"force every {S}, checkpoints every {K}",
+            "ratio": (tb / n_steps) / (ts / n_steps),
+            "backward_ms_per_step": tb * 1e3 / n_steps, "forward_ms_per_step": tf * 1e3 / n_steps,
+            "step_ms_per_step": ts * 1e3 / n_steps,
+            "dL_dE_ones": float(g[0]), "force_last": [float(x) for x in F[-1]]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,7 +218,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble", type=int, default=64, help="scenes of the ensemble run; 0: skip it")
-    ap.add_argument("--skip-headline", action="store_true", help="only the ensemble run (no JSON line)")
+    ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint timing")
+    ap.add_argument("--only", choices=("ensemble", "adjoint"), default=None,
+                    help="run only this part and print its result (not the bench line)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -191,9 +234,13 @@ def main() -> int:
     if capi.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
-    if args.skip_headline:
+    if args.only == "ensemble":
         log(f"bench: ensemble of {args.ensemble} scenes: {args.warmup} warm-up + {args.steps} timed steps")
         print(json.dumps({"ensemble": run_ensemble(args)}), flush=True)
+        return 0
+    if args.only == "adjoint":
+        log("bench: calibration adjoint (200k points, 400 substeps)")
+        print(json.dumps({"adjoint": run_adjoint(args)}), flush=True)
         return 0
     scene = replace(scenes.SCENES[args.scene], dt=args.dt)
     prms = scenes.make_params(scene)
@@ -252,6 +299,11 @@ def main() -> int:
                     "ms_per_step": el2 * 1e3 / args.steps, "device_error": err2,
                     "brick_segments_at_end": stats2["segments"]}
 
+    adjoint = None
+    if args.adjoint:
+        log("bench: calibration adjoint (200k points, 400 substeps)")
+        adjoint = run_adjoint(args)
+
     ensemble = None
     if args.ensemble > 0:
         log(f"bench: ensemble of {args.ensemble} scenes: {args.warmup} warm-up + {args.steps} timed steps")
@@ -264,7 +316,7 @@ def main() -> int:
         cpu = cpu_baseline(scene, prms, hs, args.cpu_seconds, args.cpu_max_steps, args.cpu_threads)
 
     out = {
-        "metric": "particle-updates/s and % of HBM roofline at 16M particles",
+        "metric": "particle-updates/s and % of HBM roofline at 16M particles; adjoint/forward step-time ratio",
         "value": value,
         "unit": "particle-updates/s",
         "n_gpus": 1,
@@ -293,6 +345,7 @@ def main() -> int:
                    for k, v in phases.items()},
         "unstable_dt": unstable,
         "ensemble": ensemble,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

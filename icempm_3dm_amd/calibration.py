@@ -15,7 +15,7 @@ a point does crush raises ``IcempmError`` (code ``ERR_PLASTIC``). In that
 regime the friction angle and cohesion do not act, and their gradients are
 exactly zero.
 
-``calibration_scene`` is the 200,000-point calibration workload: a
+``CALIBRATION_SCENE`` is the 200,000-point calibration workload: a
 40x25x25-cell block, 8 points per cell, on a 128x64x64 grid of h = 1/64 m,
 sphere indenter r = 0.1 m descending at 1 m/s, dt = 2e-6 s.
 """
@@ -147,8 +147,8 @@ def calibrate(model: ForceCurveModel, target: np.ndarray, theta0, iters: int = 2
               fit=(True, True, False, False)):
     """Fits the selected parameters to the target force curve with Adam on
     theta / theta0 (loss: sum of squared force differences over the samples,
-    divided by the sum of squared target forces). Returns the history of
-    (loss, theta)."""
+    divided by the sum of squared target forces); the step size halves
+    whenever the loss rises. Returns the history of (loss, theta)."""
     torch = _torch()
     theta0 = torch.tensor(np.asarray(theta0, dtype=np.float64))
     u = torch.ones(4, dtype=torch.float64, requires_grad=True)
@@ -164,6 +164,9 @@ def calibrate(model: ForceCurveModel, target: np.ndarray, theta0, iters: int = 2
         loss = ((F - tgt) ** 2).sum() / scale
         loss.backward()
         history.append((float(loss), theta.detach().numpy().copy()))
+        if len(history) > 1 and history[-1][0] > history[-2][0]:
+            for group in opt.param_groups:
+                group["lr"] *= 0.5
         opt.step()
     return history
 

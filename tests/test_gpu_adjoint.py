@@ -113,3 +113,32 @@ def test_torch_function_and_calibration_loop(require_gpu):
     assert hist[-1][0] < 0.2 * hist[0][0], [h[0] for h in hist]
     assert abs(hist[-1][1][0] / TARGET_THETA[0] - 1) < abs(theta0[0] / TARGET_THETA[0] - 1)
     m.close()
+
+
+def test_calibration_scene_gradient_and_fit(require_gpu):
+    """The 200,000-point calibration workload (128x64x64 grid, 400 substeps,
+    force every 10, checkpoints every 20): the gradient of the force misfit
+    against central differences at relative step 1e-6, then Adam from a guess
+    with E 30% high and nu 30% low reduces the misfit."""
+    from icempm_3dm_amd.calibration import CALIBRATION_SCENE
+    n_steps, S, K = 400, 10, 20
+    m = ForceCurveModel(CALIBRATION_SCENE, n_steps, S, K, theta=TARGET_THETA)
+    assert m.gpu.hssoa.size == 200_000
+    target = m.forward(TARGET_THETA)
+    theta0 = np.array(TARGET_THETA) * np.array([1.3, 0.7, 1.0, 1.0])
+    F0 = m.forward(theta0)
+    w = 2 * (F0 - target)
+    g = m.backward(w)
+    for j in range(2):
+        d = theta0[j] * 1e-6
+        vals = []
+        for sgn in (1, -1):
+            th = theta0.copy()
+            th[j] += sgn * d
+            vals.append(float(((m.forward(th) - target) ** 2).sum()))
+        fd = (vals[0] - vals[1]) / (2 * d)
+        assert abs(g[j] - fd) <= 1e-6 * abs(fd), (j, g[j], fd)
+    hist = calibrate(m, target, theta0, iters=15, lr=0.04)
+    print("misfit", [round(h[0], 6) for h in hist], "theta", hist[-1][1])
+    assert hist[-1][0] < 0.1 * hist[0][0]
+    m.close()
