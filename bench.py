@@ -18,8 +18,9 @@ checkpoints included) over a plain forward substep of the same scene.
 
 The batched ensemble (64 independent scenes of a material sweep, counts
 log-uniform in [20k, 500k], each on its own 128x64x64 sub-grid, advanced in
-the same launches) is timed last and reported as "ensemble", with its
-throughput as a fraction of the single-scene headline.
+the same launches, launch settings picked by icempm_autotune) is timed last
+and reported as "ensemble", with its throughput as a fraction of the
+single-scene headline.
 
 One step = one substep of the reference's Model3D::Step loop (reset, P2G,
 grid update, G2P with the return mapping), with the G2P of a step fused with
@@ -148,7 +149,7 @@ def run_ensemble(args) -> dict:
 
     sw = sweep_scenes(args.ensemble, seed=2024, dt=args.dt)
     ens = BatchedEnsemble(sw)
-    ens.set_fusion(args.fuse != 0)
+    tuning = ens.autotune(50)
     n = ens.n_points
     ens.step(args.warmup)
     ens.synchronize()
@@ -165,7 +166,8 @@ def run_ensemble(args) -> dict:
     counts = [s.n_points for s in sw]
     ens.close()
     return {"scenes": len(sw), "points": n, "points_min": min(counts), "points_max": max(counts),
-            "value": n * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps, "device_error": code}
+            "value": n * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps, "device_error": code,
+            "autotuned": tuning}
 
 
 def run_adjoint(args) -> dict:

@@ -132,6 +132,17 @@ class AdjointRun(C.Structure):
     _fields_ = [("n_steps", C.c_int), ("sample_every", C.c_int), ("checkpoint_every", C.c_int), ("t0", C.c_double)]
 
 
+class Tuning(C.Structure):
+    """icempm_Tuning: the launch settings icempm_autotune keeps."""
+
+    _fields_ = [("fuse", C.c_int), ("sort_period", C.c_int), ("blocks_per_cu", C.c_int),
+                ("ms_per_substep", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"fuse": bool(self.fuse), "sort_period": self.sort_period, "blocks_per_cu": self.blocks_per_cu,
+                "ms_per_substep": self.ms_per_substep}
+
+
 class IcempmError(RuntimeError):
     """A C-ABI call returned an error code (the reference throws std::runtime_error)."""
 
@@ -187,6 +198,7 @@ _PROTOS = {
     "icempm_ensemble_set_indenter": (C.c_int, [_ctxp, C.c_int, C.POINTER(Indenter)]),
     "icempm_ensemble_transfer_points": (C.c_int, [_ctxp, _dp, C.c_uint, _ip]),
     "icempm_ensemble_layout": (C.c_int, [_ctxp, C.c_int, _ip, C.POINTER(C.c_longlong)]),
+    "icempm_autotune": (C.c_int, [_ctxp, C.c_int, C.POINTER(Tuning)]),
     "icempm_adjoint_forward": (C.c_int, [_ctxp, C.POINTER(AdjointRun), _dp]),
     "icempm_adjoint_backward": (C.c_int, [_ctxp, _dp, _dp, _dp]),
     "icempm_scene_block": (C.c_int, [_dp, C.c_uint, _ip, C.c_int, _dp, C.c_double, C.c_double,

@@ -81,6 +81,7 @@ struct icempm_ctx {
     int32_t *work = nullptr; // G2P worklists: 2 counts, crushed list, spill list
     int p2g_blocks = 2048;
     int g2p_blocks = 4096;
+    int n_cu = 256;
 
     // grids: two buffers (the fused substep reads one and writes the other),
     // each with its own active-brick flags and ping-pong lists
@@ -479,6 +480,8 @@ int icempm_create(int device, icempm_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->p2g_blocks = prop.multiProcessorCount * 8;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->n_cu = prop.multiProcessorCount;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->g2p_blocks = prop.multiProcessorCount * 16;
     *out = ctx;
@@ -926,6 +929,108 @@ int icempm_remove_disabled_and_sort(icempm_ctx *ctx, int *n_points_out) {
 int icempm_sort_by_brick(icempm_ctx *ctx) {
     ICEMPM_TRY(check_ready(ctx));
     return brick_sort(ctx);
+}
+
+int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (trial_substeps < 1) return fail(ICEMPM_ERR_INVALID_ARG, "trial_substeps must be positive");
+    const int n = ctx->n;
+    icempm_Tuning keep{ctx->fuse ? 1 : 0, ctx->sort_period, std::max(1, ctx->p2g_blocks / std::max(ctx->n_cu, 1)),
+                       0.0};
+    if (n == 0) {
+        if (best) *best = keep;
+        return ICEMPM_OK;
+    }
+    // the state every trial starts from
+    const size_t lines = (size_t)ICEMPM_N_PTS_ARRAYS * ctx->pitch;
+    double *save_pts = nullptr, *save_bins = nullptr;
+    int32_t *save_slot = nullptr;
+    int save_err = 0;
+    auto release = [&]() {
+        if (save_pts) (void)hipFree(save_pts);
+        if (save_bins) (void)hipFree(save_bins);
+        if (save_slot) (void)hipFree(save_slot);
+    };
+    int rc = dev_alloc(&save_pts, lines, "autotune point copy");
+    if (rc == ICEMPM_OK) rc = dev_alloc(&save_slot, (size_t)n, "autotune slot copy");
+    if (rc == ICEMPM_OK) rc = dev_alloc(&save_bins, (size_t)std::max(ctx->nbins, 1LL), "autotune bins copy");
+    if (rc != ICEMPM_OK) {
+        release();
+        return rc;
+    }
+    const int cur = ctx->cur_pts;
+    auto copy = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    hipError_t e = copy(save_pts, ctx->pts[cur], sizeof(double) * lines);
+    if (e == hipSuccess) e = copy(save_slot, ctx->slot[cur], sizeof(int32_t) * n);
+    if (e == hipSuccess && ctx->nbins > 0) e = copy(save_bins, ctx->bins, sizeof(double) * ctx->nbins);
+    if (e == hipSuccess) e = hipMemcpyAsync(&save_err, ctx->err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        release();
+        return fail(ICEMPM_ERR_HIP, std::string("autotune save: ") + hipGetErrorString(e));
+    }
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    auto restore = [&]() -> int {
+        ctx->cur_pts = cur;
+        ICEMPM_HIP(copy(ctx->pts[cur], save_pts, sizeof(double) * lines));
+        ICEMPM_HIP(copy(ctx->slot[cur], save_slot, sizeof(int32_t) * n));
+        if (ctx->nbins > 0) ICEMPM_HIP(copy(ctx->bins, save_bins, sizeof(double) * ctx->nbins));
+        k_fill_int<<<1, 1, 0, ctx->stream>>>(ctx->err, 1, save_err);
+        ICEMPM_HIP(hipGetLastError());
+        return ICEMPM_OK;
+    };
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    (void)hipEventCreate(&ev0);
+    (void)hipEventCreate(&ev1);
+    icempm_Tuning top{keep.fuse, keep.sort_period, keep.blocks_per_cu, 1e300};
+    rc = ICEMPM_OK;
+    for (int fuse = 1; fuse >= 0 && rc == ICEMPM_OK; fuse--)
+        for (int period : {10, 20, 50})
+            for (int bpc : {4, 8, 16}) {
+                if (rc != ICEMPM_OK) break;
+                ctx->fuse = fuse != 0;
+                ctx->sort_period = period;
+                ctx->p2g_blocks = bpc * ctx->n_cu;
+                ctx->g2p_blocks = 2 * bpc * ctx->n_cu;
+                rc = restore();
+                if (rc == ICEMPM_OK) rc = brick_sort(ctx);
+                double t = 0.0;
+                if (rc == ICEMPM_OK) rc = icempm_step(ctx, 2, &t, 0);
+                if (rc != ICEMPM_OK) break;
+                float ms = 0.f;
+                if (hipEventRecord(ev0, ctx->stream) != hipSuccess) rc = fail(ICEMPM_ERR_HIP, "hipEventRecord");
+                if (rc == ICEMPM_OK) rc = icempm_step(ctx, trial_substeps, &t, 2);
+                if (rc == ICEMPM_OK && hipEventRecord(ev1, ctx->stream) != hipSuccess)
+                    rc = fail(ICEMPM_ERR_HIP, "hipEventRecord");
+                if (rc == ICEMPM_OK && (hipEventSynchronize(ev1) != hipSuccess ||
+                                        hipEventElapsedTime(&ms, ev0, ev1) != hipSuccess))
+                    rc = fail(ICEMPM_ERR_HIP, "autotune timing");
+                if (rc == ICEMPM_OK && ms / trial_substeps < top.ms_per_substep)
+                    top = icempm_Tuning{fuse, period, bpc, (double)ms / trial_substeps};
+            }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    ctx->timing = timing;
+    const icempm_Tuning use = rc == ICEMPM_OK ? top : keep;
+    ctx->fuse = use.fuse != 0;
+    ctx->sort_period = use.sort_period;
+    ctx->p2g_blocks = use.blocks_per_cu * ctx->n_cu;
+    ctx->g2p_blocks = 2 * use.blocks_per_cu * ctx->n_cu;
+    int rc2 = restore();
+    if (rc2 == ICEMPM_OK) rc2 = brick_sort(ctx);
+    if (rc2 == ICEMPM_OK) {
+        hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        if (e2 != hipSuccess) rc2 = fail(ICEMPM_ERR_HIP, std::string("autotune restore: ") + hipGetErrorString(e2));
+    }
+    release();
+    if (rc != ICEMPM_OK) return rc;
+    if (rc2 != ICEMPM_OK) return rc2;
+    ctx->substeps_since_sort = 0;
+    if (best) *best = use;
+    return ICEMPM_OK;
 }
 
 int icempm_set_sort_period(icempm_ctx *ctx, int period) {

@@ -104,6 +104,12 @@ class BatchedEnsemble:
     def set_fusion(self, on: bool) -> None:
         check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))
 
+    def autotune(self, trial_substeps: int = 10) -> dict:
+        """icempm_autotune over all scenes together (the points are left as they are)."""
+        t = capi.Tuning()
+        check(lib().icempm_autotune(self.ctx, int(trial_substeps), C.byref(t)))
+        return t.as_dict()
+
     def reset_indenter_force_accumulator(self) -> None:
         check(lib().icempm_reset_indenter_force_accumulator(self.ctx))
 

@@ -78,6 +78,13 @@ class GPUImplementation6:
     def set_fusion(self, on: bool) -> None:
         check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))
 
+    def autotune(self, trial_substeps: int = 10) -> dict:
+        """icempm_autotune: keeps the fastest fusion / sort period / workgroup count
+        for the current points (which it leaves as they are)."""
+        t = capi.Tuning()
+        check(lib().icempm_autotune(self.ctx, int(trial_substeps), C.byref(t)))
+        return t.as_dict()
+
     def transfer_ponts_to_device(self) -> None:
         """GPU_Implementation6::transfer_ponts_to_device (gpu_implementation6.cpp:404-414)."""
         check(lib().icempm_transfer_points_to_device(self.ctx, dptr(self.hssoa.host_buffer), self.hssoa.capacity, 0,

@@ -279,6 +279,22 @@ int icempm_ensemble_transfer_points(icempm_ctx *ctx, const double *hssoa, unsign
  * concatenated array. Either output may be NULL. */
 int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long long *bins_offset_out);
 
+/* Autotuner (no reference counterpart). Times trial_substeps substeps of
+ * icempm_step for each combination of G2P+P2G fusion (on/off), brick
+ * re-sort period (10, 20, 50 substeps) and workgroups per CU of the
+ * brick-segment kernels (4, 8, 16), each from the same device state,
+ * then restores the points, the indenter accumulators and the error
+ * indicator and keeps the fastest combination (written to *best if not
+ * NULL). The context's simulation time is not advanced. Needs device
+ * memory for one more copy of the points. */
+typedef struct icempm_Tuning {
+    int fuse;
+    int sort_period;
+    int blocks_per_cu;
+    double ms_per_substep;
+} icempm_Tuning;
+int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best);
+
 /* Calibration adjoint (no reference counterpart: the reference has no
  * differentiation). A run of n_steps substeps from the context's current
  * points (which it leaves untouched), with the parameters and indenter law

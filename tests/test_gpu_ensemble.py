@@ -106,3 +106,18 @@ def test_ensemble_scene_boundary_flags_and_upload_check(require_gpu):
     assert capi.lib().icempm_update_constants(ens.ctx, C.byref(params[0])) == capi.ERR_INVALID_ARG
     assert capi.lib().icempm_remove_disabled_and_sort(ens.ctx, None) == capi.ERR_INVALID_ARG
     ens.close()
+
+
+def test_autotune_keeps_the_state_and_picks_a_setting(require_gpu):
+    sc, params, indenters, clouds = _three_scenes()
+    ens = BatchedEnsemble(sc, clouds=clouds, params=params, indenters=indenters)
+    before = ens.transfer_from_device().copy()
+    pts_before = ens.hssoa.lines()[:, :ens.n_points].copy()
+    t = ens.autotune(4)
+    assert t["sort_period"] in (10, 20, 50) and t["blocks_per_cu"] in (4, 8, 16) and t["ms_per_substep"] > 0
+    after = ens.transfer_from_device()
+    assert np.array_equal(after.view(np.int64), before.view(np.int64))
+    assert np.array_equal(ens.hssoa.lines()[:, :ens.n_points].view(np.int64), pts_before.view(np.int64))
+    ens.step(3)  # the tuned context still steps
+    ens.transfer_from_device()
+    ens.close()
