@@ -9,11 +9,13 @@ dL/d(E, nu, friction angle, cohesion) for a loss L of the force samples.
 ``IndenterForceCurve`` wraps the pair as a ``torch.autograd.Function``;
 ``calibrate`` fits parameters to a target curve with Adam.
 
-The differentiated model is the elastic step: the force-curve scenes here
-raise the crush-test strengths so that no point crushes, and a run in which
-a point does crush raises ``IcempmError`` (code ``ERR_PLASTIC``). In that
-regime the friction angle and cohesion do not act, and their gradients are
-exactly zero.
+The differentiated model is the step with its plastic return mapping; the
+crush test and the return mapping's branch choices (tension cap or shear
+projection, projected or not) are held as the forward took them, so the
+gradient is exact where none of them switches under a small change of the
+parameters. ``elastic_params`` raises the crush-test strengths so that no
+point crushes: then the friction angle and cohesion do not act and their
+gradients are exactly zero, and the force curve is smooth in E and nu.
 
 ``CALIBRATION_SCENE`` is the 200,000-point calibration workload: a
 40x25x25-cell block, 8 points per cell, on a 128x64x64 grid of h = 1/64 m,
@@ -101,10 +103,10 @@ class ForceCurveModel:
 
     def backward(self, dL_dF: np.ndarray, raw: bool = False):
         """dL/d(E, nu, phi_deg, cohesion) of the last forward for dL/dF (n_samples, 3);
-        with raw, also (dL/dmu, dL/dkappa)."""
+        with raw, also dL/d(mu, kappa, DP_threshold_p, DP_tan_phi)."""
         w = np.ascontiguousarray(dL_dF, dtype=np.float64).reshape(self.n_samples, 3)
         g = np.zeros(4)
-        r = np.zeros(2)
+        r = np.zeros(4)
         check(lib().icempm_adjoint_backward(self.gpu.ctx, dptr(w), dptr(g), dptr(r)))
         return (g, r) if raw else g
 
@@ -147,8 +149,10 @@ def calibrate(model: ForceCurveModel, target: np.ndarray, theta0, iters: int = 2
               fit=(True, True, False, False)):
     """Fits the selected parameters to the target force curve with Adam on
     theta / theta0 (loss: sum of squared force differences over the samples,
-    divided by the sum of squared target forces); the step size halves
-    whenever the loss rises. Returns the history of (loss, theta)."""
+    divided by the sum of squared target forces). A step that raises the
+    loss is undone and the step size halved (Adam restarts from the best
+    point). Returns the history of (loss, theta) of every evaluation; the
+    best entry is the fit."""
     torch = _torch()
     theta0 = torch.tensor(np.asarray(theta0, dtype=np.float64))
     u = torch.ones(4, dtype=torch.float64, requires_grad=True)
@@ -157,16 +161,21 @@ def calibrate(model: ForceCurveModel, target: np.ndarray, theta0, iters: int = 2
     scale = float((tgt ** 2).sum())
     opt = torch.optim.Adam([u], lr=lr)
     history = []
+    best_loss, best_u = float("inf"), u.detach().clone()
     for _ in range(iters):
         opt.zero_grad()
         theta = theta0 * (1 + (u - 1) * mask)
         F = IndenterForceCurve.apply(theta, model)
         loss = ((F - tgt) ** 2).sum() / scale
-        loss.backward()
         history.append((float(loss), theta.detach().numpy().copy()))
-        if len(history) > 1 and history[-1][0] > history[-2][0]:
-            for group in opt.param_groups:
-                group["lr"] *= 0.5
+        if float(loss) > best_loss:
+            with torch.no_grad():
+                u.copy_(best_u)
+            lr = opt.param_groups[0]["lr"] * 0.5
+            opt = torch.optim.Adam([u], lr=lr)
+            continue
+        best_loss, best_u = float(loss), u.detach().clone()
+        loss.backward()
         opt.step()
     return history
 

@@ -25,9 +25,9 @@ STATUS_CRUSHED = 0x10000
 STATUS_DISABLED = 0x20000
 STATUS_LIQUID = 0x40000
 
-OK, ERR_INVALID_ARG, ERR_HIP, ERR_NOT_READY, ERR_CAPACITY, ERR_DEVICE_FLAG, ERR_NO_DEVICE, ERR_PLASTIC = range(8)
+OK, ERR_INVALID_ARG, ERR_HIP, ERR_NOT_READY, ERR_CAPACITY, ERR_DEVICE_FLAG, ERR_NO_DEVICE = range(7)
 DEVERR_NONFINITE = 76
-ADJ_PLASTIC, ADJ_UNSUPPORTED = 90, 91
+ADJ_UNSUPPORTED = 91
 INDENTER_CYLINDER_Z, INDENTER_SPHERE = 0, 1
 PHASES = ("sort", "reset", "p2g", "update", "g2p", "g2p2g")
 STATS = ("spilled", "sorts", "segments")

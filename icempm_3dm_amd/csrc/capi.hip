@@ -131,7 +131,7 @@ struct icempm_ctx {
         double *bar = nullptr;   // point adjoints (28-line layout)
         double *part = nullptr;  // G2P part of the point adjoints
         double *forces = nullptr;
-        double *theta = nullptr; // mu_bar, kappa_bar (2 doubles after the force samples)
+        double *theta = nullptr; // adjoints of mu, kappa, DP_threshold_p, DP_tan_phi (after the force samples)
         int *flags = nullptr;    // crushed / liquid points seen
         std::vector<double> centers, times; // indenter center and time per substep
         size_t ckpt_cap = 0, traj_cap = 0, grid_cap = 0, pt_cap = 0, f_cap = 0;
@@ -1224,8 +1224,6 @@ int adj_check(icempm_ctx *ctx) {
     ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
     if (code) return fail(ICEMPM_ERR_DEVICE_FLAG, "adjoint run: device error indicator " + std::to_string(code));
     if (flags[1]) return fail(ICEMPM_ERR_INVALID_ARG, "adjoint run: liquid points are not differentiated");
-    if (flags[0])
-        return fail(ICEMPM_ERR_PLASTIC, "adjoint run: a point crushed; the return mapping is not differentiated");
     return ICEMPM_OK;
 }
 
@@ -1285,7 +1283,7 @@ int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double
         A.pt_cap = st;
     }
     const int nsamp = R.n_steps / R.sample_every;
-    ICEMPM_TRY(adj_reserve(&A.forces, A.f_cap, 3 * (size_t)nsamp + 2, "force samples"));
+    ICEMPM_TRY(adj_reserve(&A.forces, A.f_cap, 3 * (size_t)nsamp + 4, "force samples"));
     A.theta = A.forces + 3 * nsamp;
     if (!A.flags) ICEMPM_TRY(dev_alloc(&A.flags, 2, "adjoint flags"));
     ICEMPM_HIP(hipMemsetAsync(A.flags, 0, sizeof(int) * 2, ctx->stream));
@@ -1318,7 +1316,6 @@ int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double
         ICEMPM_TRY(adj_run(ctx, seg_i * K, K, [&](int j) { return (j & 1) ? b : a; }, false, true));
         if (K & 1) std::swap(cur, nxt);
     }
-    // the crushed bit is sticky: the final state shows whether any point crushed
     ICEMPM_HIP(launch_adj_flags(adj_view(cur, n), n, A.flags, ctx->stream));
     ICEMPM_HIP(hipMemcpyAsync(forces_out, A.forces, sizeof(double) * 3 * nsamp, hipMemcpyDeviceToHost, ctx->stream));
     ICEMPM_TRY(adj_check(ctx));
@@ -1326,7 +1323,7 @@ int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double
     return ICEMPM_OK;
 }
 
-int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[2]) {
+int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[4]) {
     ICEMPM_TRY(check_ready(ctx));
     auto &A = ctx->adj;
     if (!A.have_forward) return fail(ICEMPM_ERR_NOT_READY, "icempm_adjoint_forward has not completed");
@@ -1337,7 +1334,7 @@ int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double gr
     const size_t st = (size_t)ICEMPM_N_PTS_ARRAYS * std::max(n, 1);
     ICEMPM_HIP(hipMemsetAsync(A.bar, 0, sizeof(double) * st, ctx->stream));
     ICEMPM_HIP(hipMemsetAsync(A.part, 0, sizeof(double) * st, ctx->stream));
-    ICEMPM_HIP(hipMemsetAsync(A.theta, 0, sizeof(double) * 2, ctx->stream));
+    ICEMPM_HIP(hipMemsetAsync(A.theta, 0, sizeof(double) * 4, ctx->stream));
     const PtsView bar = adj_view(A.bar, n), part = adj_view(A.part, n);
     SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
     GridState gb;
@@ -1361,13 +1358,13 @@ int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double gr
             const PtsView s_0 = adj_view(A.traj + st * j, n), s_1 = adj_view(A.traj + st * (j + 1), n);
             const double *G = A.G + (size_t)4 * gp * j, *V = A.V + (size_t)4 * gp * j;
             ICEMPM_HIP(hipMemsetAsync(A.vbar, 0, sizeof(double) * 4 * gp, ctx->stream));
-            ICEMPM_HIP(launch_adj_g2p_bwd(ctx->kp, s_0, s_1, bar, part, n, seg, V, gb, 0, ctx->p2g_blocks,
+            ICEMPM_HIP(launch_adj_g2p_bwd(ctx->kp, s_0, s_1, bar, part, n, seg, V, gb, 0, A.theta, ctx->p2g_blocks,
                                           ctx->stream));
             ICEMPM_HIP(launch_adj_update_bwd(ctx->kp, G, A.vbar, A.Gbar, &A.centers[3 * s], fbar, ctx->stream));
             ICEMPM_HIP(launch_adj_p2g_bwd(ctx->kp, s_0, A.Gbar, part, bar, A.theta, n, ctx->stream));
         }
     }
-    double th[2] = {0.0, 0.0};
+    double th[4] = {0.0, 0.0, 0.0, 0.0};
     ICEMPM_HIP(hipMemcpyAsync(th, A.theta, sizeof(th), hipMemcpyDeviceToHost, ctx->stream));
     ICEMPM_TRY(adj_check(ctx));
     // Lame chain rule (ComputeLame, parameters_sim_3d.cpp:164-169):
@@ -1379,12 +1376,14 @@ int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double gr
     const double dk_dE = 2.0 / 3.0 * dmu_dE + dl_dE, dk_dnu = 2.0 / 3.0 * dmu_dnu + dl_dnu;
     grad_out[0] = th[0] * dmu_dE + th[1] * dk_dE;
     grad_out[1] = th[0] * dmu_dnu + th[1] * dk_dnu;
-    grad_out[2] = 0.0; // friction angle and cohesion act through the return mapping only
-    grad_out[3] = 0.0;
-    if (raw_out) {
-        raw_out[0] = th[0];
-        raw_out[1] = th[1];
-    }
+    // friction angle phi (degrees) and cohesion c: DP_tan_phi = tan(phi pi / 180)
+    // (SetPhi, parameters_sim_3d.h:100), DP_threshold_p = c / DP_tan_phi
+    const double tphi = ctx->prms.DP_tan_phi, pt = ctx->prms.DP_threshold_p, c = pt * tphi;
+    const double dt_dphi = ICEMPM_PI / 180.0 * (1 + tphi * tphi);
+    grad_out[2] = (th[3] - th[2] * c / (tphi * tphi)) * dt_dphi;
+    grad_out[3] = th[2] / tphi;
+    if (raw_out)
+        for (int k = 0; k < 4; k++) raw_out[k] = th[k];
     return ICEMPM_OK;
 }
 

@@ -76,11 +76,12 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
                         hipStream_t stream);
 
 // Reverse mode of the step for the calibration adjoint (kernels_adjoint.hip);
-// grids in the step's bricked layout. V: the grid after the update (m, v);
+// grids in the step's bricked layout. theta: adjoints of (mu, kappa,
+// DP_threshold_p, DP_tan_phi), added to. V: the grid after the update (m, v);
 // gb: Vbar (v_bar in lines 1-3; lists marked, never reset); G: the grid after
 // P2G (m, mv); Gbar: (m_bar, mv_bar) over every node.
 hipError_t launch_adj_g2p_bwd(const KParams &kp, PtsView s0, PtsView s1, PtsView bar, PtsView part, int n,
-                              SegInfo seg, const double *V, GridState gb, int cur_b, int n_blocks,
+                              SegInfo seg, const double *V, GridState gb, int cur_b, double *theta, int n_blocks,
                               hipStream_t stream);
 hipError_t launch_adj_update_bwd(const KParams &kp, const double *G, const double *Vbar, double *Gbar,
                                  const double c[3], const double fbar[3], hipStream_t stream);

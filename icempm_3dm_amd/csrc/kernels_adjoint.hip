@@ -230,6 +230,139 @@ __device__ __forceinline__ void kirchhoff_adjoint(const KParams &kp, const doubl
         }
 }
 
+// Reverse of return_map (Wolper_Drucker_Prager, mpm_kernels.cuh:494-544)
+// for a point the G2P of this substep crushed or kept crushed: Ftr is the
+// trial F (A F), Jp_inv its value before the update, Fb the adjoint of the
+// returned F on entry and of Ftr on exit; th[0..3] gain the adjoints of
+// (mu, kappa, DP_threshold_p, DP_tan_phi). The returned F is an isotropic
+// function U diag(s(sigma)) V^T of Ftr: in the singular basis the diagonal
+// carries ds/dsigma, and each off-diagonal pair (i, j) the factor
+// (s_i - s_j)/(sigma_i - sigma_j) on its symmetric part and
+// (s_i + s_j)/(sigma_i + sigma_j) on its antisymmetric part.
+__device__ __noinline__ void return_map_adjoint(const KParams &kp, const double Ftr[9], double Jp_inv, double Fb[9],
+                                                double th[4]) {
+    PState p;
+#pragma unroll
+    for (int k = 0; k < 9; k++) p.Fe[k] = Ftr[k];
+    p.Jp_inv = Jp_inv;
+    p.util = 0;
+    compute_pq(kp, p);
+    double U[9], V[9], sig[3], sig2[3];
+    svd3(p.Fe, U, sig, sig2, V);
+    double G[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) acc += U[3 * a + i] * Fb[3 * a + b] * V[3 * b + j];
+            G[3 * i + j] = acc;
+        }
+    const double mu = kp.mu, kappa = kp.kappa, pt = kp.dp_threshold_p, tphi = kp.tan_phi;
+    double sh[3], ds[3][3], dth[3][4];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (m < 3) ds[k][m] = 0.0;
+            dth[k][m] = 0.0;
+        }
+    if (p.p_tr < -pt || p.Jp_inv < 1) {
+        const double Jn = kp.rm_je_new, c = kp.rm_cbrt_je_new;
+        const double dc_dJn = c / (3 * Jn);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            sh[k] = c;
+            dth[k][2] = dc_dJn / (kappa * Jn);
+            dth[k][1] = -dc_dJn * pt / (kappa * kappa * Jn);
+        }
+    } else {
+        const double pmax = kp.ice_comp, pmin = -kp.ice_tens, qmax = kp.ice_shear;
+        double q, dq_dp = 0.0, dq_dpt = 0.0, dq_dt = 0.0;
+        if (p.p_tr > pmax) {
+            q = 0.0;
+        } else {
+            const double q_dp = (p.p_tr + pt) * tphi;
+            const double root = sqrt((pmax - p.p_tr) * (p.p_tr - pmin));
+            const double q_fs = 2 * root * qmax / (pmax - pmin);
+            q = fmin(q_fs, q_dp);
+            if (q_dp < q_fs) {
+                dq_dp = tphi;
+                dq_dpt = tphi;
+                dq_dt = p.p_tr + pt;
+            } else {
+                dq_dp = qmax * (pmax + pmin - 2 * p.p_tr) / ((pmax - pmin) * root);
+            }
+        }
+        if (!(p.q_tr >= q)) return; // not projected: F unchanged
+        const double J = p.Je_tr, J23 = J * J * p.rc_Jsq * p.rc_Jsq;
+        const double coef = q * kCoeff1Inv / mu * J23;
+        const double mean = (sig2[0] + sig2[1] + sig2[2]) / 3;
+        double d[3], n[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) d[k] = sig2[k] - mean;
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) n[k] = r > 0 ? d[k] / r : 0.0;
+        const double dJ[3] = {sig[1] * sig[2], sig[0] * sig[2], sig[0] * sig[1]};
+        const double g = kCoeff1Inv / mu * J23 * (dq_dp * (-kappa * J) + (2.0 / 3.0) * q / J);
+#pragma unroll
+        for (int k = 0; k < 3; k++) sh[k] = sqrt(coef * n[k] + mean);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const double dn = r > 0 ? 2 * sig[m] / r * ((k == m ? 1.0 : 0.0) - n[k] * n[m] - 1.0 / 3.0) : 0.0;
+                ds[k][m] = (coef * dn + n[k] * g * dJ[m] + 2 * sig[m] / 3) / (2 * sh[k]);
+            }
+            const double f = n[k] / (2 * sh[k]);
+            dth[k][0] = f * (-coef / mu);
+            dth[k][1] = f * kCoeff1Inv / mu * J23 * dq_dp * (-(J * J - 1) / 2);
+            dth[k][2] = f * kCoeff1Inv / mu * J23 * dq_dpt;
+            dth[k][3] = f * kCoeff1Inv / mu * J23 * dq_dt;
+        }
+    }
+    double H[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) H[k] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) H[4 * m] += ds[k][m] * G[4 * k];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) th[t] += dth[k][t] * G[4 * k];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i + 1; j < 3; j++) {
+            double a;
+            if (fabs(sig[i] - sig[j]) > 1e-8 * fmax(1.0, fabs(sig[i])))
+                a = (sh[i] - sh[j]) / (sig[i] - sig[j]);
+            else
+                a = 0.5 * ((ds[i][i] - ds[i][j]) + (ds[j][j] - ds[j][i]));
+            const double b = (sh[i] + sh[j]) / (sig[i] + sig[j]);
+            const double sym = 0.5 * (G[3 * i + j] + G[3 * j + i]), asym = 0.5 * (G[3 * i + j] - G[3 * j + i]);
+            H[3 * i + j] = a * sym + b * asym;
+            H[3 * j + i] = a * sym - b * asym;
+        }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) acc += U[3 * a + i] * H[3 * i + j] * V[3 * b + j];
+            Fb[3 * a + b] = acc;
+        }
+}
+
 // Reverse of P2G for substep s: gathers Gbar at the point, combines with the
 // G2P part and writes the adjoints of the state at the start of s into bar;
 // mu_bar and kappa_bar are summed into theta[0..1].
@@ -381,7 +514,7 @@ __device__ __forceinline__ void adj_gather(const double w[3][3], const double dw
 
 __global__ __launch_bounds__(kP2GThreads) void k_adj_g2p_bwd(KParams kp, PtsView s0, PtsView s1, PtsView bar,
                                                              PtsView part, SegInfo seg, const double *__restrict__ V,
-                                                             GridState gb, int cur_b) {
+                                                             GridState gb, int cur_b, double *theta) {
     __shared__ P2GShared sh;
     __shared__ double sv[3][kTileAlloc];
     const int t = threadIdx.x;
@@ -403,6 +536,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_adj_g2p_bwd(KParams kp, PtsView
         bool ok = false;
         int base[3] = {0, 0, 0};
         double fx[3] = {0.0, 0.0, 0.0}, vhat[3], M[9];
+        double th[4] = {0.0, 0.0, 0.0, 0.0};
         if (i < c.end && !(double_to_bits(s0.at(ICEMPM_IDX_UTILITY, i)) & ICEMPM_STATUS_DISABLED)) {
             double pos[3];
 #pragma unroll
@@ -414,6 +548,18 @@ __global__ __launch_bounds__(kP2GThreads) void k_adj_g2p_bwd(KParams kp, PtsView
                 load_mat(s1, ICEMPM_BP00, i, C1);
                 load_mat(bar, ICEMPM_FE00, i, Fb);
                 load_mat(bar, ICEMPM_BP00, i, Cb);
+                if (double_to_bits(s1.at(ICEMPM_IDX_UTILITY, i)) & ICEMPM_STATUS_CRUSHED) {
+                    // this substep's G2P returned F to the yield surface: F'_bar -> trial F_bar
+                    double Ftr[9];
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+#pragma unroll
+                        for (int q = 0; q < 3; q++)
+                            Ftr[3 * r + q] = ((r == 0 ? 1.0 : 0.0) + dt * C1[3 * r]) * F[q] +
+                                             ((r == 1 ? 1.0 : 0.0) + dt * C1[3 * r + 1]) * F[3 + q] +
+                                             ((r == 2 ? 1.0 : 0.0) + dt * C1[3 * r + 2]) * F[6 + q];
+                    return_map_adjoint(kp, Ftr, s0.at(ICEMPM_IDX_JP_INV, i), Fb, th);
+                }
                 double xb[3];
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
@@ -468,6 +614,14 @@ __global__ __launch_bounds__(kP2GThreads) void k_adj_g2p_bwd(KParams kp, PtsView
             }
         }
         adj_record(sh, t, ok, base, fx, vhat, M, org);
+        // return-mapping parameter adjoints: wavefront sums, one atomic each
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double x = th[k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+            if ((t & 63) == 0 && x != 0.0) unsafeAtomicAdd(&theta[k], x);
+        }
         __syncthreads();
         p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), kp, gb, bl);
         if (c.last()) p2g_flush(sh, t, kp, org, bc, gb, bl);
@@ -487,7 +641,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_sum_bins(const double *bins, lo
     block_add<3>(f, out);
 }
 
-// Counts crushed and liquid points into flags[0] / flags[1].
+// Marks crushed and liquid points in flags[0] / flags[1].
 __global__ __launch_bounds__(kAdjThreads) void k_adj_flags(PtsView st, int n, int *flags) {
     const int i = blockIdx.x * kAdjThreads + threadIdx.x;
     if (i >= n) return;
@@ -502,11 +656,11 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + kAdjThreads - 1
 } // namespace
 
 hipError_t launch_adj_g2p_bwd(const KParams &kp, PtsView s0, PtsView s1, PtsView bar, PtsView part, int n,
-                              SegInfo seg, const double *V, GridState gb, int cur_b, int n_blocks,
+                              SegInfo seg, const double *V, GridState gb, int cur_b, double *theta, int n_blocks,
                               hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
-    k_adj_g2p_bwd<<<n_blocks, kP2GThreads, 0, stream>>>(kp, s0, s1, bar, part, seg, V, gb, cur_b);
+    k_adj_g2p_bwd<<<n_blocks, kP2GThreads, 0, stream>>>(kp, s0, s1, bar, part, seg, V, gb, cur_b, theta);
     return hipGetLastError();
 }
 

@@ -39,7 +39,6 @@ extern "C" {
 #define ICEMPM_ERR_CAPACITY 4      /* host or device capacity exceeded */
 #define ICEMPM_ERR_DEVICE_FLAG 5   /* the device error indicator is set */
 #define ICEMPM_ERR_NO_DEVICE 6     /* no HIP device is visible */
-#define ICEMPM_ERR_PLASTIC 7       /* adjoint run: a point crushed (no adjoint of the return mapping) */
 
 /* ---- device error indicator values -----------------------------------
  * 70..75: a particle's 3x3x3 stencil leaves the grid (mpm_kernels.cuh:70-75:
@@ -48,9 +47,7 @@ extern "C" {
  * The indicator keeps the largest code raised since the last
  * update_constants / clear. Offending particles are skipped, not written. */
 #define ICEMPM_DEVERR_NONFINITE 76
-/* Adjoint runs only: 90 a point crushed, 91 a liquid point (no adjoint of
- * the water pressure). */
-#define ICEMPM_ADJ_PLASTIC 90
+/* Adjoint runs only: 91 a liquid point (no adjoint of the water pressure). */
 #define ICEMPM_ADJ_UNSUPPORTED 91
 
 /* ---- indenter law (extension) -----------------------------------------
@@ -307,11 +304,11 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best);
  * K-substep segment from its checkpoint (keeping that segment's states and
  * grids) and runs the reverse mode through it. n_steps must be a multiple
  * of S and of K. Device memory: about (n_steps/K + K + 4) * 28 * 8 bytes per
- * point plus (7 K + 7) * 8 bytes per grid node. The model differentiated is
- * the elastic step of one scene (SetupType 0 or 1, solid points): a point
- * that crushes ends the forward with ICEMPM_ERR_PLASTIC (the return mapping
- * is not differentiated), liquid points with ICEMPM_ERR_DEVICE_FLAG (code
- * ICEMPM_ADJ_UNSUPPORTED). */
+ * point plus (8 K + 8) * 8 bytes per grid node. The model differentiated is
+ * the step of one scene (SetupType 0 or 1, solid points) with its plastic
+ * return mapping (Wolper_Drucker_Prager, mpm_kernels.cuh:494-544); the crush
+ * test and the return mapping's branch choices are held as the forward took
+ * them. Liquid points end the forward with ICEMPM_ERR_INVALID_ARG. */
 typedef struct icempm_AdjointRun {
     int n_steps;
     int sample_every;
@@ -322,10 +319,11 @@ typedef struct icempm_AdjointRun {
 int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double *forces_out);
 /* Reverse mode of the last icempm_adjoint_forward for a loss L with
  * dL/dF_k given (3 per sample, same layout as forces_out). grad_out[4]:
- * dL/d(YoungsModulus, PoissonsRatio, friction angle in degrees, cohesion);
- * the last two are exactly 0 in the elastic regime the forward guarantees.
- * raw_out[2] (may be NULL): dL/dmu and dL/dkappa. */
-int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[2]);
+ * dL/d(YoungsModulus, PoissonsRatio, friction angle in degrees, cohesion),
+ * with DP_tan_phi = tan(phi) and DP_threshold_p = cohesion / tan(phi); the
+ * last two are exactly 0 when no point is crushed. raw_out[4] (may be NULL):
+ * dL/d(mu, kappa, DP_threshold_p, DP_tan_phi). */
+int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[4]);
 
 /* Synthetic input (no reference counterpart; replaces the HDF5 point cloud of
  * SnapshotManager::LoadRawPoints for generated scenes): a block of

@@ -764,14 +764,15 @@ int oracle_step(const icempm_SimParams3D *prms, const icempm_Indenter *ind, doub
 // is the total indenter force averaged over the substeps of sample k
 // (sample_every substeps each; the per-substep force is the sum over
 // contact nodes of (v_before - v_after) m / dt, mpm_kernels.cuh:189-201).
-// Reverse mode over the elastic step: P2G (Kirchhoff stress, APIC), the
-// grid update (gravity, velocity clamp, indenter contact, walls) and G2P.
-// A point that crushes ends the run with ICEMPM_ADJ_PLASTIC: the plastic
-// return mapping is not differentiated here.
+// Reverse mode over the step: P2G (Kirchhoff stress, APIC), the grid
+// update (gravity, velocity clamp, indenter contact, walls), G2P and the
+// plastic return mapping of crushed points, with every branch (crush test,
+// tension cap or shear projection, contact, walls) held as the forward
+// took it. Liquid points end the run with code 91.
 // ====================================================================
 namespace {
 
-constexpr int kAdjPlastic = 90;
+constexpr int kAdjUnsupported = 91;
 
 inline void bspline_dw(double f, double dw[3]) {
     dw[0] = -(1.5 - f);
@@ -893,6 +894,113 @@ void kirchhoff_adjoint(double mu, double kappa, const Mat3 &F, const Mat3 &tb, M
     for (int k = 0; k < 9; k++) Fbar.a[k] += mu * s * TSF.a[k] + cJ * cof.a[k];
 }
 
+// Reverse of wolper_drucker_prager for a point crushed in this substep:
+// Ftr is the trial F (A F), Jp_inv its value before the update, Fb the
+// adjoint of the returned F on entry and of Ftr on exit; th[0..3] gain the
+// adjoints of (mu, kappa, DP_threshold_p, DP_tan_phi). The returned F is an
+// isotropic function U diag(s(sigma)) V^T of Ftr: in the singular basis the
+// diagonal carries ds/dsigma, and each off-diagonal pair (i, j) the factors
+// (s_i - s_j)/(sigma_i - sigma_j) on its symmetric part and
+// (s_i + s_j)/(sigma_i + sigma_j) on its antisymmetric part.
+void return_map_adjoint(const icempm_SimParams3D &P, const Mat3 &Ftr, double Jp_inv, Mat3 &Fb, double th[4]) {
+    Point p;
+    p.Fe = Ftr;
+    p.Jp_inv = Jp_inv;
+    compute_pq(p, P.kappa, P.mu);
+    compute_svd(p, P.kappa, P.mu);
+    const double mu = P.mu, kappa = P.kappa, pt = P.DP_threshold_p, tphi = P.DP_tan_phi;
+    const double *sig = p.vSigma;
+    // Uhat = U^T Fb V
+    Mat3 G;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) acc += p.U(a, i) * Fb(a, b) * p.V(b, j);
+            G(i, j) = acc;
+        }
+    double sh[3], ds[3][3] = {{0}}, dth[3][4] = {{0}};
+    if (p.p_tr < -pt || p.Jp_inv < 1) {
+        const double Jn = std::sqrt(-2. * (-pt) / kappa + 1.);
+        const double c = std::cbrt(Jn);
+        const double dc_dJn = c / (3 * Jn);
+        for (int k = 0; k < 3; k++) {
+            sh[k] = c;
+            dth[k][2] = dc_dJn / (kappa * Jn);           // dJn/dpt = 1 / (kappa Jn)
+            dth[k][1] = -dc_dJn * pt / (kappa * kappa * Jn); // dJn/dkappa
+        }
+    } else {
+        const double pmax = P.IceCompressiveStrength, pmin = -P.IceTensileStrength, qmax = P.IceShearStrength;
+        double q, dq_dp = 0, dq_dpt = 0, dq_dt = 0;
+        if (p.p_tr > pmax) {
+            q = 0;
+        } else {
+            const double q_dp = (p.p_tr + pt) * tphi;
+            const double root = std::sqrt((pmax - p.p_tr) * (p.p_tr - pmin));
+            const double q_fs = 2 * root * qmax / (pmax - pmin);
+            q = std::fmin(q_fs, q_dp);
+            if (q_dp < q_fs) {
+                dq_dp = tphi;
+                dq_dpt = tphi;
+                dq_dt = p.p_tr + pt;
+            } else {
+                dq_dp = qmax * (pmax + pmin - 2 * p.p_tr) / ((pmax - pmin) * root);
+            }
+        }
+        if (!(p.q_tr >= q)) return; // not projected: F unchanged
+        const double J = p.Je_tr, J23 = std::cbrt(J * J);
+        const double K = q * kCoeff1Inv / mu; // coef = K J^(2/3)
+        const double coef = K * J23;
+        double s2[3], d[3];
+        for (int k = 0; k < 3; k++) s2[k] = sig[k] * sig[k];
+        const double mean = (s2[0] + s2[1] + s2[2]) / 3;
+        for (int k = 0; k < 3; k++) d[k] = s2[k] - mean;
+        const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double n[3];
+        for (int k = 0; k < 3; k++) n[k] = r > 0 ? d[k] / r : 0.0;
+        const double dJ[3] = {sig[1] * sig[2], sig[0] * sig[2], sig[0] * sig[1]};
+        // dcoef/dsigma_m = (kCoeff1Inv/mu) J^(2/3) dJ_m (dq/dp dp/dJ + (2/3) q / J), dp/dJ = -kappa J
+        const double g = kCoeff1Inv / mu * J23 * (dq_dp * (-kappa * J) + (2.0 / 3.0) * q / J);
+        for (int k = 0; k < 3; k++) sh[k] = std::sqrt(coef * n[k] + mean);
+        for (int k = 0; k < 3; k++) {
+            for (int m = 0; m < 3; m++) {
+                const double dn = r > 0 ? 2 * sig[m] / r * ((k == m ? 1.0 : 0.0) - n[k] * n[m] - 1.0 / 3.0) : 0.0;
+                ds[k][m] = (coef * dn + n[k] * g * dJ[m] + 2 * sig[m] / 3) / (2 * sh[k]);
+            }
+            const double f = n[k] / (2 * sh[k]);
+            dth[k][0] = f * (-coef / mu);
+            dth[k][1] = f * kCoeff1Inv / mu * J23 * dq_dp * (-(J * J - 1) / 2);
+            dth[k][2] = f * kCoeff1Inv / mu * J23 * dq_dpt;
+            dth[k][3] = f * kCoeff1Inv / mu * J23 * dq_dt;
+        }
+    }
+    Mat3 H = zero3();
+    for (int m = 0; m < 3; m++)
+        for (int k = 0; k < 3; k++) H(m, m) += ds[k][m] * G(k, k);
+    for (int k = 0; k < 3; k++)
+        for (int t = 0; t < 4; t++) th[t] += dth[k][t] * G(k, k);
+    for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++) {
+            double a;
+            if (std::fabs(sig[i] - sig[j]) > 1e-8 * std::max(1.0, std::fabs(sig[i])))
+                a = (sh[i] - sh[j]) / (sig[i] - sig[j]);
+            else
+                a = 0.5 * ((ds[i][i] - ds[i][j]) + (ds[j][j] - ds[j][i]));
+            const double b = (sh[i] + sh[j]) / (sig[i] + sig[j]);
+            const double sym = 0.5 * (G(i, j) + G(j, i)), asym = 0.5 * (G(i, j) - G(j, i));
+            H(i, j) = a * sym + b * asym;
+            H(j, i) = a * sym - b * asym;
+        }
+    // Fb = U H V^T
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double acc = 0;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) acc += p.U(a, i) * H(i, j) * p.V(b, j);
+            Fb(a, b) = acc;
+        }
+}
+
 } // namespace
 
 extern "C" {
@@ -932,14 +1040,13 @@ int oracle_adj_forward(const icempm_SimParams3D *prms, const icempm_Indenter *in
             for (int i = 0; i < 3; i++) forces[3 * (s / sample_every) + i] += fs[i] / sample_every;
         code = oracle_g2p(prms, soa.data(), pitch, n, grid.data(), 0, 1);
         if (code) return code;
-        for (int i = 0; i < n; i++)
-            if (util_bits(soa[i]) & ICEMPM_STATUS_CRUSHED) return kAdjPlastic;
     }
     return 0;
 }
 
 int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
                         int n, int n_steps, int sample_every, const double *dL_dF, double *grad_mu_kappa) {
+    // grad_mu_kappa: dL/d(mu, kappa, DP_threshold_p, DP_tan_phi)
     const icempm_SimParams3D &P = *prms;
     const size_t nn = (size_t)P.GridXTotal * P.GridY * P.GridZ;
     const size_t lines = (size_t)pitch * ICEMPM_N_PTS_ARRAYS;
@@ -965,7 +1072,7 @@ int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *i
         code = oracle_g2p(prms, cur, pitch, n, grid.data(), 0, 1);
         if (code) return code;
         for (int i = 0; i < n; i++)
-            if (util_bits(cur[i]) & ICEMPM_STATUS_CRUSHED) return kAdjPlastic;
+            if (util_bits(cur[i]) & ICEMPM_STATUS_LIQUID) return kAdjUnsupported;
     }
     // reverse
     std::vector<PtState> bar(n);
@@ -976,6 +1083,7 @@ int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *i
     }
     std::vector<double> gm(nn * 4), vgrid(nn * 3), vbar(nn * 3), gbar(nn * 4);
     double mub = 0.0, kappab = 0.0;
+    double th[4] = {0.0, 0.0, 0.0, 0.0}; // return-map part: mu, kappa, DP_threshold_p, DP_tan_phi
     for (int s = n_steps - 1; s >= 0; s--) {
         const double *st = &states[(size_t)s * lines];
         const double *nx = &states[(size_t)(s + 1) * lines];
@@ -1008,6 +1116,11 @@ int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *i
             weights_and_check(P, S0.x, W);
             Mat3 A = eye3();
             for (int k = 0; k < 9; k++) A.a[k] += dt * S1.C.a[k];
+            if (util_bits(nx[p]) & ICEMPM_STATUS_CRUSHED) {
+                // the G2P of this substep returned F to the yield surface
+                const Mat3 Ftr = mul(A, S0.F);
+                return_map_adjoint(P, Ftr, st[p + (size_t)pitch * ICEMPM_IDX_JP_INV], B.F, th);
+            }
             Mat3 chat = B.C;
             const Mat3 FbFt = mul_abt(B.F, S0.F);
             for (int k = 0; k < 9; k++) chat.a[k] += dt * FbFt.a[k];
@@ -1135,8 +1248,10 @@ int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *i
             kirchhoff_adjoint(mu, kappa, S0.F, tb, B.F, mub, kappab);
         }
     }
-    grad_mu_kappa[0] = mub;
-    grad_mu_kappa[1] = kappab;
+    grad_mu_kappa[0] = mub + th[0];
+    grad_mu_kappa[1] = kappab + th[1];
+    grad_mu_kappa[2] = th[2];
+    grad_mu_kappa[3] = th[3];
     return 0;
 }
 

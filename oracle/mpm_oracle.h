@@ -75,12 +75,12 @@ int oracle_step(const icempm_SimParams3D *prms, const icempm_Indenter *ind, doub
 /* Calibration adjoint (no reference counterpart). Forward: n_steps substeps
  * of oracle_step's model from soa0 (left untouched), force samples
  * F_k = (1/S) sum of the total indenter force over substeps [kS, (k+1)S),
- * S = sample_every, written to forces (3 per sample). Returns 90 if a point
- * crushes (the return mapping is not differentiated), else an error code. */
+ * S = sample_every, written to forces (3 per sample). Returns an error code. */
 int oracle_adj_forward(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
                        int n, int n_steps, int sample_every, double *forces);
-/* Reverse mode of the same run for L with dL/dF_k given (3 per sample):
- * grad_mu_kappa[0] = dL/dmu, grad_mu_kappa[1] = dL/dkappa. */
+/* Reverse mode of the same run for L with dL/dF_k given (3 per sample),
+ * the plastic return mapping included: grad_mu_kappa[0..3] = dL/d(mu, kappa,
+ * DP_threshold_p, DP_tan_phi). Returns 91 for liquid points. */
 int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
                         int n, int n_steps, int sample_every, const double *dL_dF, double *grad_mu_kappa);
 

@@ -174,9 +174,9 @@ def adj_forward(p: SimParams3D, ind: Indenter, soa: np.ndarray, pitch: int, n: i
 
 def adj_gradient(p: SimParams3D, ind: Indenter, soa: np.ndarray, pitch: int, n: int, n_steps: int,
                  sample_every: int, dL_dF: np.ndarray):
-    """Reverse mode of adj_forward's run: (code, dL/dmu, dL/dkappa)."""
-    g = np.zeros(2)
+    """Reverse mode of adj_forward's run: (code, dL/dmu, dL/dkappa, dL/dDP_threshold_p, dL/dDP_tan_phi)."""
+    g = np.zeros(4)
     w = np.ascontiguousarray(dL_dF, dtype=np.float64)
     code = lib().oracle_adj_gradient(C.byref(p), C.byref(ind), _d(soa), pitch, n, n_steps, sample_every, _d(w),
                                      _d(g))
-    return code, g[0], g[1]
+    return code, g[0], g[1], g[2], g[3]

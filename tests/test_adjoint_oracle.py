@@ -65,7 +65,7 @@ def test_forward_forces_match_step_accumulators():
 def test_lame_gradient_matches_central_differences(name, idx):
     p, ind, hs = _setup()
     w = np.random.default_rng(3).standard_normal((N_STEPS // S, 3))
-    code, gmu, gk = orc.adj_gradient(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S, w)
+    code, gmu, gk, _, _ = orc.adj_gradient(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S, w)
     assert code == 0
     g = (gmu, gk)[idx]
     base = getattr(p, name)
@@ -83,8 +83,9 @@ def test_young_and_poisson_gradient_through_lame():
     theta = (8e9, 0.3, 55.0, 1.5e6)
     p, ind, hs = _setup(theta)
     w = np.random.default_rng(4).standard_normal((N_STEPS // S, 3))
-    code, gmu, gk = orc.adj_gradient(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S, w)
+    code, gmu, gk, gpt, gtan = orc.adj_gradient(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S, w)
     assert code == 0
+    assert gpt == 0.0 and gtan == 0.0  # no point crushes
     g = np.array([gmu, gk]) @ lame_chain(theta[0], theta[1])
     for j in range(2):
         d = theta[j] * 1e-6
@@ -104,9 +105,38 @@ def test_young_and_poisson_gradient_through_lame():
         assert _loss(q, ind, hs, w) == _loss(p, ind, hs, w)
 
 
-def test_crushing_run_is_refused():
-    p = scenes.make_params(replace(TINY, indenter_speed=20.0))
-    ind = capi.Indenter.make(*replace(TINY, indenter_speed=20.0).indenter())
-    hs = scenes.make_points(TINY)
-    code, _ = orc.adj_forward(p, ind, hs.host_buffer, hs.capacity, hs.size, 200, 10)
-    assert code == 90
+PLASTIC = replace(TINY, name="adj_tiny_plastic", indenter_speed=5.0)
+
+
+def plastic_setup():
+    """Every point crushed from the start (the crush test cannot switch),
+    the reference's crush strengths and cohesion 0.3e6 Pa: under the
+    indenter the return mapping projects onto the Drucker-Prager cone."""
+    p = scenes.make_params(PLASTIC)
+    p.DP_threshold_p = 0.3e6 / p.DP_tan_phi
+    ind = capi.Indenter.make(*PLASTIC.indenter())
+    hs = scenes.make_points(PLASTIC)
+    hs.utility()[:hs.size] |= capi.STATUS_CRUSHED
+    return p, ind, hs
+
+
+@pytest.mark.parametrize("name,idx", [("mu", 0), ("kappa", 1), ("DP_threshold_p", 2), ("DP_tan_phi", 3)])
+def test_plastic_gradient_matches_central_differences(name, idx):
+    """With the return mapping active the loss is piecewise smooth: the
+    projection switches on at q = q_max and min(q_fs, q_dp) switches
+    branch. The scene is mild enough (40 substeps, 5 m/s) that no point
+    crosses a switch within the 1e-6 steps; the bar is 1e-5 relative."""
+    p, ind, hs = plastic_setup()
+    w = np.random.default_rng(3).standard_normal((N_STEPS // S, 3))
+    code, *g = orc.adj_gradient(p, ind, hs.host_buffer, hs.capacity, hs.size, N_STEPS, S, w)
+    assert code == 0
+    assert g[2] != 0.0 and g[3] != 0.0
+    base = getattr(p, name)
+    d = base * 1e-6
+    vals = []
+    for sgn in (1, -1):
+        q = p.copy()
+        setattr(q, name, base + sgn * d)
+        vals.append(_loss(q, ind, hs, w))
+    fd = (vals[0] - vals[1]) / (2 * d)
+    assert abs(g[idx] - fd) <= 1e-5 * abs(fd), (name, g[idx], fd)
