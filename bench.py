@@ -10,6 +10,10 @@ material and indenter but no time step; the headline uses dt = 2e-6 s
 block blows apart, every point crushes and spreads over ~4x more grid
 bricks) is timed after it and reported as "unstable_dt".
 
+The same workload at 27 and 64 points per cell (134x67x66 and 100x50x50
+cells on the same grid, 200 timed steps each) is reported as
+"points_per_cell".
+
 The calibration adjoint (reverse mode of the elastic step on the
 200,000-point calibration scene: 40x25x25 cells on a 128x64x64 grid, 400
 substeps, force sampled every 10, checkpoints every 20) is timed after it and
@@ -170,6 +174,30 @@ def run_ensemble(args) -> dict:
             "autotuned": tuning}
 
 
+def run_ppc(args, name: str) -> dict:
+    """The 16M-point throughput workload at another point density (same
+    grid, fewer cells): particle-updates/s and the fused kernel's HBM
+    roofline fraction."""
+    from dataclasses import replace
+    from icempm_3dm_amd import scenes
+
+    scene = replace(scenes.SCENES[name], dt=args.dt)
+    hs = scenes.make_points(scene)
+    n = hs.size
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup = min(args.steps, 200), min(args.warmup, 20)
+    elapsed, timings, err, active_nodes, stats = run_steps(scene, hs, sub, args.dt, True)
+    ms, cnt = timings.get("g2p2g", (0.0, 0))
+    out = {"workload": f"{name}: {n} points ({scene.ppc_axis ** 3}/cell) on a 512x256x256 grid, dt={args.dt:g}",
+           "particles": n, "steps": sub.steps, "value": n * sub.steps / elapsed,
+           "ms_per_step": elapsed * 1e3 / sub.steps, "active_nodes": active_nodes, "device_error": err}
+    if cnt:
+        b = (B_G2P_PT + B_P2G_PT) * n + (B_G2P_NODE + B_P2G_NODE) * active_nodes
+        gbs = b / (ms / cnt * 1e-3) / 1e9
+        out.update({"k_g2p2g_avg_ms": ms / cnt, "k_g2p2g_alg_GBs": gbs, "k_g2p2g_frac": gbs / HBM_PEAK_GBS})
+    return out
+
+
 def run_adjoint(args) -> dict:
     """Forward with checkpoints and backward of the calibration run, and the
     plain step on the same scene; times per substep."""
@@ -221,6 +249,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble", type=int, default=64, help="scenes of the ensemble run; 0: skip it")
     ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint timing")
+    ap.add_argument("--ppc-sweep", type=int, default=1, help="0: skip the 27 and 64 points-per-cell runs")
     ap.add_argument("--only", choices=("ensemble", "adjoint"), default=None,
                     help="run only this part and print its result (not the bench line)")
     args = ap.parse_args()
@@ -301,6 +330,13 @@ def main() -> int:
                     "ms_per_step": el2 * 1e3 / args.steps, "device_error": err2,
                     "brick_segments_at_end": stats2["segments"]}
 
+    ppc = None
+    if args.ppc_sweep:
+        ppc = {}
+        for name in ("block_16m_ppc27", "block_16m_ppc64"):
+            log(f"bench: {name}")
+            ppc[name] = run_ppc(args, name)
+
     adjoint = None
     if args.adjoint:
         log("bench: calibration adjoint (200k points, 400 substeps)")
@@ -346,6 +382,7 @@ def main() -> int:
         "phases": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in phases.items()},
         "unstable_dt": unstable,
+        "points_per_cell": ppc,
         "ensemble": ensemble,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
