@@ -123,6 +123,10 @@ def run_steps(scene, hs, args, dt: float, timed: bool):
     t_sim = g.step(args.warmup, t_sim, step)
     step += args.warmup
     g.synchronize()
+    launch = {"fuse": args.fuse != 0, "sort_period": args.sort_period, "blocks_per_cu": 8, "autotuned": False}
+    if args.autotune:
+        # launch settings from the autotuner, timed on the warmed-up state (which it restores)
+        launch = dict(g.autotune(50), autotuned=True)
     if timed:
         g.enable_timing(True)
         g.reset_timings()
@@ -144,6 +148,7 @@ def run_steps(scene, hs, args, dt: float, timed: bool):
         active_nodes = int(np.count_nonzero(grid[0]))
         del grid
     g.close()
+    stats["launch"] = launch
     return elapsed, timings, err, active_nodes, stats
 
 
@@ -250,6 +255,8 @@ def main() -> int:
     ap.add_argument("--ensemble", type=int, default=64, help="scenes of the ensemble run; 0: skip it")
     ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint timing")
     ap.add_argument("--ppc-sweep", type=int, default=1, help="0: skip the 27 and 64 points-per-cell runs")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: launch settings from icempm_autotune (else --fuse / --sort-period, 8 workgroups per CU)")
     ap.add_argument("--only", choices=("ensemble", "adjoint"), default=None,
                     help="run only this part and print its result (not the bench line)")
     args = ap.parse_args()
@@ -328,7 +335,7 @@ def main() -> int:
         el2, _, err2, _, stats2 = run_steps(scene, hs, args, args.unstable_dt, False)
         unstable = {"dt": args.unstable_dt, "value": n * args.steps / el2,
                     "ms_per_step": el2 * 1e3 / args.steps, "device_error": err2,
-                    "brick_segments_at_end": stats2["segments"]}
+                    "brick_segments_at_end": stats2["segments"], "launch": stats2["launch"]}
 
     ppc = None
     if args.ppc_sweep:
@@ -369,13 +376,15 @@ def main() -> int:
         "config": {
             "workload": f"{scene.name}: {n} points ({scene.ppc_axis ** 3}/cell) on a "
                         f"{scene.grid[0]}x{scene.grid[1]}x{scene.grid[2]} grid, h=1/64, dt={args.dt:g}, "
-                        f"sphere indenter, G2P+P2G fusion {'on' if args.fuse else 'off'}, "
-                        f"brick re-sort every {args.sort_period} steps",
+                        f"sphere indenter, G2P+P2G fusion {'on' if stats['launch']['fuse'] else 'off'}, "
+                        f"brick re-sort every {stats['launch']['sort_period']} steps, "
+                        f"{stats['launch']['blocks_per_cu']} workgroups per CU",
             "particles": n,
             "grid": list(scene.grid),
             "dt": args.dt,
             "active_nodes": active_nodes,
             "brick_segments_at_end": stats["segments"],
+            "launch": stats["launch"],
             "device_error": err,
         },
         "roofline": roofline,

@@ -136,6 +136,7 @@ struct icempm_ctx {
         std::vector<double> centers, times; // indenter center and time per substep
         size_t ckpt_cap = 0, traj_cap = 0, grid_cap = 0, pt_cap = 0, f_cap = 0;
         bool have_forward = false;
+        bool any_crushed = false; // a point of the last forward run is crushed
     } adj;
 
     // timing
@@ -1224,6 +1225,7 @@ int adj_check(icempm_ctx *ctx) {
     ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
     if (code) return fail(ICEMPM_ERR_DEVICE_FLAG, "adjoint run: device error indicator " + std::to_string(code));
     if (flags[1]) return fail(ICEMPM_ERR_INVALID_ARG, "adjoint run: liquid points are not differentiated");
+    ctx->adj.any_crushed = flags[0] != 0; // sticky bit: the final state shows every crushed point
     return ICEMPM_OK;
 }
 
@@ -1358,8 +1360,8 @@ int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double gr
             const PtsView s_0 = adj_view(A.traj + st * j, n), s_1 = adj_view(A.traj + st * (j + 1), n);
             const double *G = A.G + (size_t)4 * gp * j, *V = A.V + (size_t)4 * gp * j;
             ICEMPM_HIP(hipMemsetAsync(A.vbar, 0, sizeof(double) * 4 * gp, ctx->stream));
-            ICEMPM_HIP(launch_adj_g2p_bwd(ctx->kp, s_0, s_1, bar, part, n, seg, V, gb, 0, A.theta, ctx->p2g_blocks,
-                                          ctx->stream));
+            ICEMPM_HIP(launch_adj_g2p_bwd(ctx->kp, s_0, s_1, bar, part, n, seg, V, gb, 0, A.theta, A.any_crushed,
+                                          ctx->p2g_blocks, ctx->stream));
             ICEMPM_HIP(launch_adj_update_bwd(ctx->kp, G, A.vbar, A.Gbar, &A.centers[3 * s], fbar, ctx->stream));
             ICEMPM_HIP(launch_adj_p2g_bwd(ctx->kp, s_0, A.Gbar, part, bar, A.theta, n, ctx->stream));
         }

@@ -512,6 +512,9 @@ __device__ __forceinline__ void adj_gather(const double w[3][3], const double dw
             }
 }
 
+// kPlastic: some point is crushed in the run (else the return-mapping
+// reverse, and its registers, stay out of the kernel).
+template <bool kPlastic>
 __global__ __launch_bounds__(kP2GThreads) void k_adj_g2p_bwd(KParams kp, PtsView s0, PtsView s1, PtsView bar,
                                                              PtsView part, SegInfo seg, const double *__restrict__ V,
                                                              GridState gb, int cur_b, double *theta) {
@@ -548,7 +551,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_adj_g2p_bwd(KParams kp, PtsView
                 load_mat(s1, ICEMPM_BP00, i, C1);
                 load_mat(bar, ICEMPM_FE00, i, Fb);
                 load_mat(bar, ICEMPM_BP00, i, Cb);
-                if (double_to_bits(s1.at(ICEMPM_IDX_UTILITY, i)) & ICEMPM_STATUS_CRUSHED) {
+                if (kPlastic && (double_to_bits(s1.at(ICEMPM_IDX_UTILITY, i)) & ICEMPM_STATUS_CRUSHED)) {
                     // this substep's G2P returned F to the yield surface: F'_bar -> trial F_bar
                     double Ftr[9];
 #pragma unroll
@@ -616,7 +619,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_adj_g2p_bwd(KParams kp, PtsView
         adj_record(sh, t, ok, base, fx, vhat, M, org);
         // return-mapping parameter adjoints: wavefront sums, one atomic each
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < (kPlastic ? 4 : 0); k++) {
             double x = th[k];
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
@@ -656,11 +659,14 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + kAdjThreads - 1
 } // namespace
 
 hipError_t launch_adj_g2p_bwd(const KParams &kp, PtsView s0, PtsView s1, PtsView bar, PtsView part, int n,
-                              SegInfo seg, const double *V, GridState gb, int cur_b, double *theta, int n_blocks,
-                              hipStream_t stream) {
+                              SegInfo seg, const double *V, GridState gb, int cur_b, double *theta, bool plastic,
+                              int n_blocks, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
-    k_adj_g2p_bwd<<<n_blocks, kP2GThreads, 0, stream>>>(kp, s0, s1, bar, part, seg, V, gb, cur_b, theta);
+    if (plastic)
+        k_adj_g2p_bwd<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, s0, s1, bar, part, seg, V, gb, cur_b, theta);
+    else
+        k_adj_g2p_bwd<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, s0, s1, bar, part, seg, V, gb, cur_b, theta);
     return hipGetLastError();
 }
 
