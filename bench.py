@@ -118,12 +118,14 @@ def run_steps(scene, hs, args, dt: float, timed: bool):
     g.set_indenter(shape, center, vel)
     g.set_sort_period(args.sort_period)
     g.set_fusion(args.fuse != 0)
+    g.set_blocks_per_cu(args.blocks_per_cu)
     g.transfer_ponts_to_device()
     t_sim, step = 0.0, 0
     t_sim = g.step(args.warmup, t_sim, step)
     step += args.warmup
     g.synchronize()
-    launch = {"fuse": args.fuse != 0, "sort_period": args.sort_period, "blocks_per_cu": 8, "autotuned": False}
+    launch = {"fuse": args.fuse != 0, "sort_period": args.sort_period, "blocks_per_cu": args.blocks_per_cu,
+              "autotuned": False}
     if args.autotune:
         # launch settings from the autotuner, timed on the warmed-up state (which it restores)
         launch = dict(g.autotune(50), autotuned=True)
@@ -256,7 +258,8 @@ def main() -> int:
     ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint timing")
     ap.add_argument("--ppc-sweep", type=int, default=1, help="0: skip the 27 and 64 points-per-cell runs")
     ap.add_argument("--autotune", type=int, default=1,
-                    help="1: launch settings from icempm_autotune (else --fuse / --sort-period, 8 workgroups per CU)")
+                    help="1: launch settings from icempm_autotune (else --fuse / --sort-period / --blocks-per-cu)")
+    ap.add_argument("--blocks-per-cu", type=int, default=8)
     ap.add_argument("--only", choices=("ensemble", "adjoint"), default=None,
                     help="run only this part and print its result (not the bench line)")
     args = ap.parse_args()

@@ -199,6 +199,7 @@ _PROTOS = {
     "icempm_ensemble_transfer_points": (C.c_int, [_ctxp, _dp, C.c_uint, _ip]),
     "icempm_ensemble_layout": (C.c_int, [_ctxp, C.c_int, _ip, C.POINTER(C.c_longlong)]),
     "icempm_autotune": (C.c_int, [_ctxp, C.c_int, C.POINTER(Tuning)]),
+    "icempm_set_blocks_per_cu": (C.c_int, [_ctxp, C.c_int]),
     "icempm_adjoint_forward": (C.c_int, [_ctxp, C.POINTER(AdjointRun), _dp]),
     "icempm_adjoint_backward": (C.c_int, [_ctxp, _dp, _dp, _dp]),
     "icempm_scene_block": (C.c_int, [_dp, C.c_uint, _ip, C.c_int, _dp, C.c_double, C.c_double,

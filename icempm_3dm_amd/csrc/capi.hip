@@ -1034,6 +1034,14 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     return ICEMPM_OK;
 }
 
+int icempm_set_blocks_per_cu(icempm_ctx *ctx, int blocks_per_cu) {
+    if (!ctx || blocks_per_cu < 1 || blocks_per_cu > 64)
+        return fail(ICEMPM_ERR_INVALID_ARG, "blocks_per_cu must be in [1, 64]");
+    ctx->p2g_blocks = blocks_per_cu * ctx->n_cu;
+    ctx->g2p_blocks = 2 * blocks_per_cu * ctx->n_cu;
+    return ICEMPM_OK;
+}
+
 int icempm_set_sort_period(icempm_ctx *ctx, int period) {
     if (!ctx || period < 0) return fail(ICEMPM_ERR_INVALID_ARG, "bad sort period");
     ctx->sort_period = period;

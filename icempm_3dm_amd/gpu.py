@@ -78,6 +78,9 @@ class GPUImplementation6:
     def set_fusion(self, on: bool) -> None:
         check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))
 
+    def set_blocks_per_cu(self, blocks_per_cu: int) -> None:
+        check(lib().icempm_set_blocks_per_cu(self.ctx, int(blocks_per_cu)))
+
     def autotune(self, trial_substeps: int = 10) -> dict:
         """icempm_autotune: keeps the fastest fusion / sort period / workgroup count
         for the current points (which it leaves as they are)."""

@@ -284,6 +284,9 @@ int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long l
  * indicator and keeps the fastest combination (written to *best if not
  * NULL). The context's simulation time is not advanced. Needs device
  * memory for one more copy of the points. */
+/* Workgroups per CU of the brick-segment kernels (P2G, fused G2P+P2G; G2P
+ * uses twice as many): 8 after icempm_create. */
+int icempm_set_blocks_per_cu(icempm_ctx *ctx, int blocks_per_cu);
 typedef struct icempm_Tuning {
     int fuse;
     int sort_period;

@@ -117,7 +117,10 @@ def test_plastic_gradient_matches_oracle_and_finite_differences(require_gpu):
     for k in range(4):
         assert abs(raw[k] - g_ref[k]) <= 1e-6 * abs(g_ref[k]), (k, raw, g_ref)
     assert g[2] != 0.0 and g[3] != 0.0
-    # central differences of the device forward in E, nu, phi, c (relative step 1e-6)
+    # central differences of the device forward in E, nu, phi, c (relative step
+    # 1e-6). Bar 5e-5: the device forward's own rounding (atomic summation
+    # order) moves the loss by ~1e-12 relative, which a 1e-6 step turns into
+    # ~1e-5 relative noise in the cohesion difference quotient
     theta = np.array(PLASTIC_THETA)
     for j in range(4):
         d = theta[j] * 1e-6
@@ -127,7 +130,7 @@ def test_plastic_gradient_matches_oracle_and_finite_differences(require_gpu):
             th[j] += sgn * d
             vals.append(float((w * m.forward(th)).sum()))
         fd = (vals[0] - vals[1]) / (2 * d)
-        assert abs(g[j] - fd) <= 1e-5 * abs(fd), (j, g[j], fd)
+        assert abs(g[j] - fd) <= 5e-5 * abs(fd), (j, g[j], fd)
     print("plastic dL/d(E, nu, phi, c)", g)
     m.close()
 
