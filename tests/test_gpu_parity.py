@@ -429,3 +429,69 @@ def test_snapshot_resume_matches_continuous_run(require_gpu, tmp_path):
     assert np.array_equal(Lb[capi.IDX_UTILITY].view(np.int64), La[capi.IDX_UTILITY].view(np.int64))
     m1.gpu.close()
     m2.gpu.close()
+
+
+# ------------------------------------------------------------------ long runs
+def test_long_run_force_curve_matches_oracle(require_gpu):
+    """The 100,000-point block (64x32x32 grid, sphere at 1 m/s) for 500
+    substeps at dt = 2e-6 s in cycles of 10 (Model3D::Step, model_3d.cpp:
+    35-60): the device's indenter force per cycle against the oracle's
+    within 1% of the largest force, and the total mass on the grid equal to
+    the point mass after the run (to round-off)."""
+    sc = replace(SMALL, dt=2e-6)
+    p = scenes.make_params(sc)
+    shape, center, vel = sc.indenter()
+    ind = capi.Indenter.make(shape, center, vel)
+    hs = scenes.make_points(sc)
+    ref = hs.copy()
+    g = _gpu_with(p, hs, indenter=(shape, center, vel))
+    g.set_sort_period(20)
+    n_cycles, every = 50, 10
+    f_dev, f_ref = [], []
+    t_dev, t_ref = 0.0, 0.0
+    for k in range(n_cycles):
+        g.reset_indenter_force_accumulator()
+        t_dev = g.step(every, t_dev, k * every)
+        f_dev.append(g.read_indenter_bins().reshape(-1, 3).sum(0) / every)
+        bins = np.zeros(p.IndenterArraySize())
+        code, t_ref = orc.step(p, ind, ref.host_buffer, ref.capacity, ref.size, orc.grid_zeros(p), bins, every, t_ref,
+                               k * every, 16)
+        assert code == 0
+        f_ref.append(bins.reshape(-1, 3).sum(0) / every)
+    f_dev, f_ref = np.array(f_dev), np.array(f_ref)
+    scale = np.abs(f_ref).max()
+    assert scale > 0
+    err = np.abs(f_dev - f_ref).max() / scale
+    print("force curve: max |F| %.4g N, max deviation %.3g of it" % (scale, err))
+    assert err <= 0.01
+    g.reset_grid()
+    g.p2g()
+    grid = g.download_grid()
+    M = p.ParticleMass * hs.size
+    assert abs(grid[0].sum() - M) <= 1e-12 * M
+    g.close()
+
+
+def test_free_flight_conserves_momentum(require_gpu):
+    """No gravity, the indenter outside the grid and the block clear of the
+    walls: with APIC transfers the total momentum of the points is unchanged
+    by a step (P2G keeps it, the stress and affine terms sum to zero by the
+    first moment of the weights, G2P gives it back) — to round-off, over 40
+    substeps with random velocities, affine fields and deformations."""
+    sc = replace(SMALL, dt=2e-6, gravity=0.0, cells=(12, 10, 12))
+    p = scenes.make_params(sc)
+    hs = scenes.perturbed_points(sc, seed=9, amp_F=2e-3, amp_v=0.5, amp_B=20.0, crushed_fraction=0.0)
+    L = hs.lines()
+    n = hs.size
+    L[capi.POSX + 1, :n] += 8 * p.cellsize  # lift the block off the sticky band: 10 nodes clear of every wall
+    far = (10.0, 10.0, 10.0)
+    g = _gpu_with(p, hs, indenter=(capi.INDENTER_SPHERE, far, (0.0, 0.0, 0.0)))
+    m = p.ParticleMass
+    mom0 = m * L[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
+    scale = m * np.abs(L[capi.VELX:capi.VELX + 3, :n]).sum(axis=1)
+    g.step(40, 0.0, 0)
+    back = _download(g)
+    mom1 = m * back.lines()[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
+    assert back.lines()[capi.POSX + 1, :n].min() > 3 * p.cellsize
+    assert np.all(np.abs(mom1 - mom0) <= 1e-12 * scale), (mom0, mom1, scale)
+    g.close()
