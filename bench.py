@@ -24,7 +24,8 @@ The batched ensemble (64 independent scenes of a material sweep, counts
 log-uniform in [20k, 500k], each on its own 128x64x64 sub-grid, advanced in
 the same launches, launch settings picked by icempm_autotune) is timed last
 and reported as "ensemble", with its throughput as a fraction of the
-single-scene headline.
+single-scene headline: launched substep by substep ("launched_value") and
+replayed from hipGraphs captured per run between re-sorts ("value").
 
 One step = one substep of the reference's Model3D::Step loop (reset, P2G,
 grid update, G2P with the return mapping), with the G2P of a step fused with
@@ -162,13 +163,21 @@ def run_ensemble(args) -> dict:
     ens = BatchedEnsemble(sw)
     tuning = ens.autotune(50)
     n = ens.n_points
-    ens.step(args.warmup)
-    ens.synchronize()
-    t0 = time.perf_counter()
-    ens.step(args.steps)
-    ens.synchronize()
-    elapsed = time.perf_counter() - t0
-    code = 0
+
+    def timed(steps):
+        ens.step(args.warmup)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.step(steps)
+        ens.synchronize()
+        return time.perf_counter() - t0
+
+    # launched substep by substep, then replayed from captured hipGraphs
+    # (one graph per run between re-sorts, captured during the warm-up)
+    elapsed_launch = timed(args.steps)
+    ens.set_graph(True)
+    elapsed = timed(args.steps)
+    captures = ens.graph_captures()[0]
     import ctypes as C
     from icempm_3dm_amd import capi
     c = C.c_int(0)
@@ -177,7 +186,9 @@ def run_ensemble(args) -> dict:
     counts = [s.n_points for s in sw]
     ens.close()
     return {"scenes": len(sw), "points": n, "points_min": min(counts), "points_max": max(counts),
-            "value": n * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps, "device_error": code,
+            "value": n * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps, "mode": "hipGraph replay",
+            "graph_captures": captures, "launched_value": n * args.steps / elapsed_launch,
+            "launched_ms_per_step": elapsed_launch * 1e3 / args.steps, "device_error": code,
             "autotuned": tuning}
 
 
@@ -227,16 +238,24 @@ def run_adjoint(args) -> dict:
     t = gpu.step(50, 0.0, 0)
     gpu.synchronize()
     t0 = time.perf_counter()
-    gpu.step(n_steps, t, 50)
+    t = gpu.step(n_steps, t, 50)
     gpu.synchronize()
     ts = time.perf_counter() - t0
+    # the same substeps replayed from captured hipGraphs
+    gpu.set_graph(True)
+    t = gpu.step(50, t, 50 + n_steps)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    gpu.step(n_steps, t, 100 + n_steps)
+    gpu.synchronize()
+    tg = time.perf_counter() - t0
     n = m.gpu.hssoa.size
     m.close()
     return {"workload": f"{CALIBRATION_SCENE.name}: {n} points on a 128x64x64 grid, {n_steps} substeps, dt=2e-6, "
                         f"force every {S}, checkpoints every {K}",
             "ratio": (tb / n_steps) / (ts / n_steps),
             "backward_ms_per_step": tb * 1e3 / n_steps, "forward_ms_per_step": tf * 1e3 / n_steps,
-            "step_ms_per_step": ts * 1e3 / n_steps,
+            "step_ms_per_step": ts * 1e3 / n_steps, "step_graph_ms_per_step": tg * 1e3 / n_steps,
             "dL_dE_ones": float(g[0]), "force_last": [float(x) for x in F[-1]]}
 
 

@@ -29,7 +29,7 @@ OK, ERR_INVALID_ARG, ERR_HIP, ERR_NOT_READY, ERR_CAPACITY, ERR_DEVICE_FLAG, ERR_
 DEVERR_NONFINITE = 76
 ADJ_UNSUPPORTED = 91
 INDENTER_CYLINDER_Z, INDENTER_SPHERE = 0, 1
-PHASES = ("sort", "reset", "p2g", "update", "g2p", "g2p2g")
+PHASES = ("sort", "reset", "p2g", "update", "g2p", "g2p2g", "graph")
 STATS = ("spilled", "sorts", "segments")
 
 
@@ -186,6 +186,8 @@ _PROTOS = {
     "icempm_sort_by_brick": (C.c_int, [_ctxp]),
     "icempm_set_sort_period": (C.c_int, [_ctxp, C.c_int]),
     "icempm_set_fusion": (C.c_int, [_ctxp, C.c_int]),
+    "icempm_set_graph": (C.c_int, [_ctxp, C.c_int]),
+    "icempm_graph_captures": (C.c_int, [_ctxp, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
     "icempm_read_stats": (C.c_int, [_ctxp, _i64p, C.c_int]),
     "icempm_cell_keys": (C.c_int, [_ctxp, C.c_int, _u32p]),
     "icempm_device_order": (C.c_int, [_ctxp, _i32p]),

@@ -52,6 +52,22 @@ struct TimedLaunch {
     hipEvent_t start, stop;
 };
 
+// Everything a captured run of substeps has baked into its launches: the
+// kernel parameter block, point count, launch sizes, and the host-side
+// ping-pong state (grid, brick lists, point buffer) it starts from. Compared
+// bytewise, so it is zero-filled before the fields are set.
+struct GraphKey {
+    KParams kp;
+    int n, p2g_blocks, g2p_blocks, fuse, sort_first, run;
+    int g, cur0, cur1, cur_pts, clock_slot;
+};
+
+struct GraphEntry {
+    GraphKey key;
+    hipGraphExec_t exec;
+    int g, cur0, cur1, cur_pts, clock_slot; // the ping-pong state after a replay
+};
+
 } // namespace
 
 struct icempm_ctx {
@@ -139,6 +155,15 @@ struct icempm_ctx {
         bool any_crushed = false; // a point of the last forward run is crushed
     } adj;
 
+    // captured runs of substeps (icempm_set_graph): the device clock that
+    // gives them t and the p/q record flag, and the cache of executable graphs
+    StepClock *clock = nullptr; // two slots
+    int clock_slot = 0;         // the slot holding the current time
+    bool graph = false;
+    bool capturing = false;
+    std::vector<GraphEntry> graphs;
+    long long graph_captures = 0;
+
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -197,13 +222,13 @@ struct PhaseTimer {
     int phase;
     hipEvent_t start = nullptr;
     PhaseTimer(icempm_ctx *c, int ph) : ctx(c), phase(ph) {
-        if (ctx->timing) {
+        if (ctx->timing && !ctx->capturing) {
             start = take_event(ctx);
             if (start && hipEventRecord(start, ctx->stream) != hipSuccess) start = nullptr;
         }
     }
     int finish() {
-        if (!ctx->timing || !start) return ICEMPM_OK;
+        if (!ctx->timing || ctx->capturing || !start) return ICEMPM_OK;
         hipEvent_t stop = take_event(ctx);
         if (!stop) return fail(ICEMPM_ERR_HIP, "hipEventCreate failed");
         ICEMPM_HIP(hipEventRecord(stop, ctx->stream));
@@ -302,10 +327,21 @@ template <typename T> int dev_alloc(T **p, size_t count, const char *what) {
     return ICEMPM_OK;
 }
 
+// Destroys the cached graphs once the stream has finished their replays.
+void drop_graphs(icempm_ctx *ctx) {
+    if (ctx->graphs.empty()) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->graphs) (void)hipGraphExecDestroy(e.exec);
+    ctx->graphs.clear();
+}
+
 void free_all(icempm_ctx *ctx) {
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
+    drop_graphs(ctx);
+    f(ctx->clock);
+    ctx->clock = nullptr;
     for (int i = 0; i < 2; i++) {
         f(ctx->pts[i]);
         f(ctx->slot[i]);
@@ -393,36 +429,157 @@ int do_p2g(icempm_ctx *ctx, int gi) {
     return tm.finish();
 }
 
-int do_update(icempm_ctx *ctx, int gi, double cx, double cy, double cz, double t) {
+void indenter_center(const icempm_ctx *ctx, double t, double c[3]) {
+    for (int d = 0; d < 3; d++) c[d] = ctx->ind.center_initial[d] + t * ctx->ind.velocity[d];
+}
+
+// Grid update of grid gi, with the reset of grid reset_gi (-1: none) and the
+// clear of the G2P spill count in the same launch. clocked: t and the
+// indenter center come from the device clock, which the launch advances
+// from slot clock_slot to the other slot; otherwise from the arguments.
+int do_update(icempm_ctx *ctx, int gi, double cx, double cy, double cz, double t, int reset_gi = -1,
+              bool clocked = false) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_UPDATE);
-    ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(gi), ctx->grids[gi].cur, cx, cy, cz, t, ctx->bins,
+    UpdateExtras x;
+    if (reset_gi >= 0) {
+        x.reset = ctx->gstate(reset_gi);
+        x.reset_cur = ctx->grids[reset_gi].cur;
+    }
+    x.zero = ctx->work;
+    if (clocked) {
+        x.clk_in = ctx->clock + ctx->clock_slot;
+        x.clk_out = ctx->clock + (ctx->clock_slot ^ 1);
+    }
+    ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(gi), ctx->grids[gi].cur, cx, cy, cz, t, ctx->bins, x,
                                    ctx->stream));
+    if (reset_gi >= 0) ctx->grids[reset_gi].cur ^= 1;
+    if (clocked) ctx->clock_slot ^= 1;
     return tm.finish();
 }
 
-int do_g2p(icempm_ctx *ctx, int gi, int record_pq) {
+// G2P of the current points from grid gi. count_zeroed: the grid update just
+// before cleared the spill count. clocked: the record flag is the clock's.
+int do_g2p(icempm_ctx *ctx, int gi, int record_pq, bool count_zeroed = false, bool clocked = false) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_G2P);
     SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
     ICEMPM_HIP(launch_g2p(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[gi].grid, record_pq, ctx->err,
-                          ctx->work, ctx->g2p_blocks, ctx->stream));
+                          ctx->work, ctx->g2p_blocks, clocked ? ctx->clock + ctx->clock_slot : nullptr, count_zeroed,
+                          ctx->stream));
     ctx->substeps_since_sort++;
     return tm.finish();
 }
 
-// G2P of one substep from grid ga fused with P2G of the next into grid gb.
-// Only icempm_step calls it, always followed by another fused pass or the
-// closing G2P, which recompute velocity and C: those lines are not stored.
-int do_g2p2g(icempm_ctx *ctx, int ga, int gb, int record_pq) {
+// G2P of one substep from grid ga fused with P2G of the next into grid gb
+// (reset, and the spill count cleared, by the grid update before it). Only
+// icempm_step calls it, always followed by another fused pass or the closing
+// G2P, which recompute velocity and C: those lines are not stored.
+int do_g2p2g(icempm_ctx *ctx, int ga, int gb, int record_pq, bool clocked) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_G2P2G);
     SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
     ICEMPM_HIP(launch_g2p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[ga].grid, ctx->gstate(gb),
-                            ctx->grids[gb].cur, record_pq, 0, ctx->err, ctx->work, ctx->p2g_blocks, ctx->stream));
+                            ctx->grids[gb].cur, record_pq, 0, ctx->err, ctx->work, ctx->p2g_blocks,
+                            clocked ? ctx->clock + ctx->clock_slot : nullptr, true, ctx->stream));
     ctx->substeps_since_sort++;
     return tm.finish();
 }
 
-void indenter_center(const icempm_ctx *ctx, double t, double c[3]) {
-    for (int d = 0; d < 3; d++) c[d] = ctx->ind.center_initial[d] + t * ctx->ind.velocity[d];
+// One run of `run` substeps over the current brick segments (Model3D::Step's
+// substep loop, model_3d.cpp:35-51): reset, P2G and grid update of the first
+// substep, then G2P of substep k fused with P2G of substep k+1 on the other
+// grid (or, unfused, run = 1), and the closing G2P. Each grid update also
+// resets the grid the next fused pass scatters into. The host advances t;
+// step0 is the substep number of the first substep (p/q recorded when it is
+// a multiple of UpdateEveryNthStep). clocked: the launches take t and the
+// record flags from the device clock instead, so that they can be captured
+// once and replayed.
+int step_run(icempm_ctx *ctx, int run, double &t, long long step0, bool clocked) {
+    const int every = ctx->prms.UpdateEveryNthStep;
+    auto rec = [&](int k) { return ((step0 + k) % every) == 0 ? 1 : 0; };
+    double c[3];
+    t += ctx->prms.InitialTimeStep;
+    indenter_center(ctx, t, c);
+    ICEMPM_TRY(do_reset_grid(ctx, ctx->g));
+    ICEMPM_TRY(do_p2g(ctx, ctx->g));
+    ICEMPM_TRY(do_update(ctx, ctx->g, c[0], c[1], c[2], t, run > 1 ? ctx->g ^ 1 : -1, clocked));
+    for (int k = 0; k + 1 < run; k++) {
+        const int nxt = ctx->g ^ 1;
+        ICEMPM_TRY(do_g2p2g(ctx, ctx->g, nxt, rec(k), clocked));
+        t += ctx->prms.InitialTimeStep;
+        indenter_center(ctx, t, c);
+        // grid g was last read by the pass above: reset it for the next one
+        ICEMPM_TRY(do_update(ctx, nxt, c[0], c[1], c[2], t, k + 2 < run ? ctx->g : -1, clocked));
+        ctx->g = nxt;
+    }
+    return do_g2p(ctx, ctx->g, rec(run - 1), true, clocked);
+}
+
+GraphKey graph_key(const icempm_ctx *ctx, bool sort_first, int run) {
+    GraphKey k;
+    std::memset(&k, 0, sizeof(k));
+    std::memcpy(&k.kp, &ctx->kp, sizeof(KParams));
+    k.n = ctx->n;
+    k.p2g_blocks = ctx->p2g_blocks;
+    k.g2p_blocks = ctx->g2p_blocks;
+    k.fuse = ctx->fuse ? 1 : 0;
+    k.sort_first = sort_first ? 1 : 0;
+    k.run = run;
+    k.g = ctx->g;
+    k.cur0 = ctx->grids[0].cur;
+    k.cur1 = ctx->grids[1].cur;
+    k.cur_pts = ctx->cur_pts;
+    k.clock_slot = ctx->clock_slot;
+    return k;
+}
+
+// Captures [brick sort +] one run of substeps on the clock into an
+// executable graph. The host-side state the capture walked through is put
+// back; the entry records where a replay leaves it.
+int capture_run(icempm_ctx *ctx, const GraphKey &key, bool sort_first, int run, GraphEntry **out) {
+    if (ctx->graphs.size() >= 16) drop_graphs(ctx);
+    const int g = ctx->g, cur0 = ctx->grids[0].cur, cur1 = ctx->grids[1].cur, cur_pts = ctx->cur_pts;
+    const int slot = ctx->clock_slot;
+    const long long since = ctx->substeps_since_sort, sorts = ctx->n_sorts;
+    ICEMPM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    double t = 0.0;
+    int rc = sort_first ? brick_sort(ctx) : ICEMPM_OK;
+    if (rc == ICEMPM_OK) rc = step_run(ctx, run, t, 0, true);
+    ctx->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    GraphEntry ent;
+    ent.key = key;
+    ent.exec = nullptr;
+    ent.g = ctx->g;
+    ent.cur0 = ctx->grids[0].cur;
+    ent.cur1 = ctx->grids[1].cur;
+    ent.cur_pts = ctx->cur_pts;
+    ent.clock_slot = ctx->clock_slot;
+    ctx->g = g;
+    ctx->clock_slot = slot;
+    ctx->grids[0].cur = cur0;
+    ctx->grids[1].cur = cur1;
+    ctx->cur_pts = cur_pts;
+    ctx->substeps_since_sort = since;
+    ctx->n_sorts = sorts;
+    if (rc != ICEMPM_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(ICEMPM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiateWithFlags(&ent.exec, graph, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(ICEMPM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    ctx->graphs.push_back(ent);
+    ctx->graph_captures++;
+    *out = &ctx->graphs.back();
+    return ICEMPM_OK;
+}
+
+GraphEntry *find_graph(icempm_ctx *ctx, const GraphKey &key) {
+    for (auto &e : ctx->graphs)
+        if (std::memcmp(&e.key, &key, sizeof(GraphKey)) == 0) return &e;
+    return nullptr;
 }
 
 __global__ void k_fill_double(double *p, long long n, double v) {
@@ -549,6 +706,9 @@ int alloc_common(icempm_ctx *ctx, const icempm_SimParams3D &prms, int n_points_c
     ctx->nbins = nbins;
     ICEMPM_TRY(dev_alloc(&ctx->bins, (size_t)ctx->nbins, "indenter bins"));
     ICEMPM_TRY(dev_alloc(&ctx->err, 1, "error indicator"));
+    ICEMPM_TRY(dev_alloc(&ctx->clock, 2, "substep clock"));
+    ICEMPM_HIP(launch_clock_set(ctx->clock, 0.0, 0, 1, ctx->stream));
+    ctx->clock_slot = 0;
     for (int i = 0; i < 2; i++) ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(i), nb, ctx->stream));
     ctx->g = 0;
     ICEMPM_TRY(zero_bins(ctx));
@@ -839,34 +999,61 @@ int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int si
     const int every = ctx->prms.UpdateEveryNthStep;
     if (every <= 0) return fail(ICEMPM_ERR_INVALID_ARG, "UpdateEveryNthStep must be positive");
     double t = *simulation_time;
-    double c[3];
+    if (ctx->graph && n_substeps > 0) {
+        ICEMPM_HIP(launch_clock_set(ctx->clock, t, simulation_step, every, ctx->stream));
+        ctx->clock_slot = 0;
+    }
     int s = 0;
     while (s < n_substeps) {
-        if (ctx->sort_period > 0 && ctx->substeps_since_sort >= ctx->sort_period) ICEMPM_TRY(brick_sort(ctx));
+        const bool sort_now = ctx->sort_period > 0 && ctx->substeps_since_sort >= ctx->sort_period;
         // substeps until the next regrouping
         int run = n_substeps - s;
-        if (ctx->sort_period > 0) run = std::min<long long>(run, ctx->sort_period - ctx->substeps_since_sort);
+        if (ctx->sort_period > 0)
+            run = (int)std::min<long long>(run, ctx->sort_period - (sort_now ? 0 : ctx->substeps_since_sort));
         if (!ctx->fuse) run = 1;
-        // first substep of the run: reset, P2G, grid update
-        t += ctx->prms.InitialTimeStep;
-        indenter_center(ctx, t, c);
-        ICEMPM_TRY(do_reset_grid(ctx, ctx->g));
-        ICEMPM_TRY(do_p2g(ctx, ctx->g));
-        ICEMPM_TRY(do_update(ctx, ctx->g, c[0], c[1], c[2], t));
-        // then G2P of substep k fused with P2G of substep k+1 on the other grid
-        for (int k = 0; k + 1 < run; k++) {
-            const int nxt = ctx->g ^ 1;
-            ICEMPM_TRY(do_reset_grid(ctx, nxt));
-            ICEMPM_TRY(do_g2p2g(ctx, ctx->g, nxt, ((simulation_step + s + k) % every) == 0 ? 1 : 0));
-            t += ctx->prms.InitialTimeStep;
-            indenter_center(ctx, t, c);
-            ICEMPM_TRY(do_update(ctx, nxt, c[0], c[1], c[2], t));
-            ctx->g = nxt;
+        if (ctx->graph) {
+            const GraphKey key = graph_key(ctx, sort_now, run);
+            GraphEntry *ent = find_graph(ctx, key);
+            if (!ent) ICEMPM_TRY(capture_run(ctx, key, sort_now, run, &ent));
+            PhaseTimer tm(ctx, ICEMPM_PHASE_GRAPH);
+            ICEMPM_HIP(hipGraphLaunch(ent->exec, ctx->stream));
+            ctx->g = ent->g;
+            ctx->grids[0].cur = ent->cur0;
+            ctx->grids[1].cur = ent->cur1;
+            ctx->cur_pts = ent->cur_pts;
+            ctx->clock_slot = ent->clock_slot;
+            if (sort_now) {
+                ctx->substeps_since_sort = 0;
+                ctx->n_sorts++;
+            }
+            ctx->substeps_since_sort += run;
+            for (int k = 0; k < run; k++) t += ctx->prms.InitialTimeStep; // the clock's additions
+            ICEMPM_TRY(tm.finish());
+        } else {
+            if (sort_now) ICEMPM_TRY(brick_sort(ctx));
+            ICEMPM_TRY(step_run(ctx, run, t, (long long)simulation_step + s, false));
         }
-        ICEMPM_TRY(do_g2p(ctx, ctx->g, ((simulation_step + s + run - 1) % every) == 0 ? 1 : 0));
         s += run;
     }
     *simulation_time = t;
+    return ICEMPM_OK;
+}
+
+int icempm_set_graph(icempm_ctx *ctx, int on) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    ctx->graph = on != 0;
+    if (!ctx->graph) {
+        ICEMPM_TRY(set_device(ctx));
+        ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+        drop_graphs(ctx);
+    }
+    return ICEMPM_OK;
+}
+
+int icempm_graph_captures(icempm_ctx *ctx, long long *captures_out, int *cached_out) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    if (captures_out) *captures_out = ctx->graph_captures;
+    if (cached_out) *cached_out = (int)ctx->graphs.size();
     return ICEMPM_OK;
 }
 
@@ -972,8 +1159,10 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
         release();
         return fail(ICEMPM_ERR_HIP, std::string("autotune save: ") + hipGetErrorString(e));
     }
-    const bool timing = ctx->timing;
+    // trials launch directly: a capture per trial would be timed with it
+    const bool timing = ctx->timing, graph = ctx->graph;
     ctx->timing = false;
+    ctx->graph = false;
     auto restore = [&]() -> int {
         ctx->cur_pts = cur;
         ICEMPM_HIP(copy(ctx->pts[cur], save_pts, sizeof(double) * lines));
@@ -1015,6 +1204,7 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     ctx->timing = timing;
+    ctx->graph = graph;
     const icempm_Tuning use = rc == ICEMPM_OK ? top : keep;
     ctx->fuse = use.fuse != 0;
     ctx->sort_period = use.sort_period;
@@ -1171,16 +1361,6 @@ int adj_reserve(double **p, size_t &cap, size_t count, const char *what) {
 
 PtsView adj_view(double *base, int n) { return PtsView{base, (long long)n}; }
 
-// The lines G2P reads (utility, Jp_inv, position, Fe) from one state to the next.
-int adj_copy_g2p_inputs(icempm_ctx *ctx, const double *from, double *to) {
-    const int n = ctx->adj.n;
-    const int spans[3][2] = {{ICEMPM_IDX_UTILITY, 1}, {ICEMPM_IDX_JP_INV, 4}, {ICEMPM_FE00, 9}};
-    for (const auto &sp : spans)
-        ICEMPM_HIP(hipMemcpyAsync(to + (size_t)sp[0] * n, from + (size_t)sp[0] * n, sizeof(double) * n * sp[1],
-                                  hipMemcpyDeviceToDevice, ctx->stream));
-    return ICEMPM_OK;
-}
-
 // Substeps s0 .. s0 + count - 1 of the step's own kernels from the state in
 // slot(0); the state after substep s0 + j goes to slot(j + 1). P2G of each
 // substep after the first is fused with the G2P before it (velocity and C
@@ -1205,22 +1385,22 @@ int adj_run(icempm_ctx *ctx, int s0, int count, Slot slot, bool keep, bool recor
             ICEMPM_HIP(hipMemcpyAsync(A.G + (size_t)4 * gp * j, ctx->grids[ga].grid, sizeof(double) * 4 * gp,
                                       hipMemcpyDeviceToDevice, ctx->stream));
         ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(ga), ctx->grids[ga].cur, c[0], c[1], c[2], A.times[s],
-                                       ctx->bins, ctx->stream));
+                                       ctx->bins, UpdateExtras{}, ctx->stream));
         if (record && (s + 1) % S == 0)
             ICEMPM_HIP(launch_sum_bins(ctx->bins, ctx->nbins, 1.0 / S, A.forces + 3 * (s / S), ctx->stream));
         if (keep)
             ICEMPM_HIP(hipMemcpyAsync(A.V + (size_t)4 * gp * j, ctx->grids[ga].grid, sizeof(double) * 4 * gp,
                                       hipMemcpyDeviceToDevice, ctx->stream));
-        ICEMPM_TRY(adj_copy_g2p_inputs(ctx, slot(j), slot(j + 1)));
-        const PtsView out = adj_view(slot(j + 1), A.n);
+        // G2P reads slot(j) and writes the new state to slot(j + 1)
+        const PtsView in = adj_view(slot(j), A.n), out = adj_view(slot(j + 1), A.n);
         if (j + 1 < count) {
             ICEMPM_TRY(do_reset_grid(ctx, gb));
             ICEMPM_HIP(launch_g2p2g(ctx->kp, out, A.n, seg, ctx->grids[ga].grid, ctx->gstate(gb), ctx->grids[gb].cur, 0,
-                                    1, ctx->err, ctx->work, ctx->p2g_blocks, ctx->stream));
+                                    1, ctx->err, ctx->work, ctx->p2g_blocks, nullptr, false, ctx->stream, &in));
             std::swap(ga, gb);
         } else {
             ICEMPM_HIP(launch_g2p(ctx->kp, out, A.n, seg, ctx->grids[ga].grid, 0, ctx->err, ctx->work,
-                                  ctx->g2p_blocks, ctx->stream));
+                                  ctx->g2p_blocks, nullptr, false, ctx->stream, &in));
         }
     }
     return ICEMPM_OK;

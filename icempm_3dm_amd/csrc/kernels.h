@@ -39,6 +39,23 @@ hipError_t launch_iota(int32_t *v, int n, hipStream_t stream);
 hipError_t exclusive_scan(const int32_t *in, int32_t *out, int n, int32_t *scratch, hipStream_t stream);
 size_t exclusive_scan_scratch_ints(int n);
 
+// ---- substep clock (kernels_grid.hip) --------------------------------------
+// Simulation time and substep counter on the device, for step sequences that
+// are captured once and replayed (a hipGraph cannot take new kernel arguments
+// per replay). Two slots: the grid update of a substep reads slot `in`
+// (t and step of the previous substep), uses t + dt (the host's own sequence
+// of additions, so t is the same double) and writes slot `out`: t, step + 1
+// and record = (step % every == 0), the flag the substep's G2P reads.
+// Kernels given a clock read t / record from it instead of their t /
+// record_pq arguments.
+struct StepClock {
+    double t;
+    long long step;
+    int every;
+    int record;
+};
+hipError_t launch_clock_set(StepClock *clk, double t, long long step, int every, hipStream_t stream);
+
 // ---- grid (kernels_grid.hip) ------------------------------------------------
 struct GridState {
     double *grid;          // 4 lines of grid_pitch doubles
@@ -50,9 +67,21 @@ struct GridState {
 hipError_t launch_reset_grid(const KParams &kp, GridState gs, int cur, hipStream_t stream);
 // Zeroes the whole grid, all flags and both counts (after allocation).
 hipError_t launch_clear_grid_all(const KParams &kp, GridState gs, long long nbricks, hipStream_t stream);
+// Work the substep loop folds into the grid update's launch (all optional):
+// the reset of another grid (launch_reset_grid of `reset` with list
+// reset_cur), zeroing *zero (the G2P spill count), and the clock advance
+// (clk_in -> clk_out; the update then takes t from the clock and the
+// indenter center as ind_c0 + t * ind_vel).
+struct UpdateExtras {
+    GridState reset{};
+    int reset_cur = 0;
+    int *zero = nullptr;
+    const StepClock *clk_in = nullptr;
+    StepClock *clk_out = nullptr;
+};
 // partition_kernel_update_nodes over the bricks of list `cur`.
 hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double cx, double cy, double cz, double t,
-                               double *bins, hipStream_t stream);
+                               double *bins, const UpdateExtras &x, hipStream_t stream);
 // Copy into the reference's x-major node order.
 hipError_t launch_grid_to_xmajor(const KParams &kp, const double *grid, double *out, hipStream_t stream);
 
@@ -66,14 +95,21 @@ hipError_t launch_p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, GridSt
                       int *err, int n_blocks, hipStream_t stream);
 // G2P including the return mapping of crushed particles. lists: 1 + n ints
 // of scratch (spill count, spill list).
+// src: read the point state from *src and write the new state to pts (every
+// line G2P writes, and the utility line of disabled points); the default is
+// in place.
 hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid, int record_pq, int *err,
-                      int *lists, int n_blocks, hipStream_t stream);
+                      int *lists, int n_blocks, const StepClock *clk, bool count_zeroed, hipStream_t stream,
+                      const PtsView *src = nullptr);
 // G2P of one substep reading grid_a fused with P2G of the next substep into
 // gs_b (kernels_g2p2g.hip). lists: 1 + n ints of scratch. store_vc = 0
 // leaves the velocity and C lines unwritten (the next reader is a G2P).
+// With clk (launch_g2p too), record_pq is clk->record. count_zeroed: the
+// spill count (lists[0]) is already zero (cleared by the preceding update).
 hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid_a, GridState gs_b,
                         int cur_b, int record_pq, int store_vc, int *err, int *lists, int n_blocks,
-                        hipStream_t stream);
+                        const StepClock *clk, bool count_zeroed, hipStream_t stream,
+                        const PtsView *src = nullptr);
 
 // Reverse mode of the step for the calibration adjoint (kernels_adjoint.hip);
 // grids in the step's bricked layout. theta: adjoints of (mu, kappa,

@@ -17,10 +17,14 @@ __global__ void k_zero_count(int *c) { *c = 0; }
 // drifted outside its segment's tile since the sort goes to the spill list.
 constexpr int kG2PWaves = kP2GThreads / 64;
 
-template <bool kEns>
+// kOOP: the point state is read from `in` and the new state written to pts;
+// otherwise in is pts.
+template <bool kEns, bool kOOP>
 __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, SegInfo seg,
-                                                     const double *__restrict__ grid, int record_pq, int *err,
-                                                     int *spill, int *spill_count) {
+                                                     const double *__restrict__ grid, int record_pq,
+                                                     const StepClock *clk, int *err, int *spill, int *spill_count,
+                                                     PtsView in) {
+    if (clk) record_pq = clk->record;
     __shared__ double sv_all[kG2PWaves][3][kTileAlloc];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double(*sv)[kTileAlloc] = sv_all[wave];
@@ -41,7 +45,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, Se
             const int i = c0 + lane;
             PointState s;
             s.util = ICEMPM_STATUS_DISABLED;
-            if (i < end) load_g2p_inputs(pts, i, s);
+            if (i < end) load_g2p_state(in, i, s);
+            if (kOOP && (s.util & ICEMPM_STATUS_DISABLED) && i < end)
+                pts.at(ICEMPM_IDX_UTILITY, i) = bits_to_double(s.util);
             bool spilled = false;
             if (!(s.util & ICEMPM_STATUS_DISABLED)) {
                 int base[3];
@@ -55,7 +61,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, Se
                         double w3[3][3], d[3][3];
                         stencil_weights(fx, w3, d);
                         gather_tile(sv, tile_index(rx, ry, rz), w3, d, c4h, s.vel, s.Bp);
-                        g2p_finish<true>(ks, pts, i, s, record_pq);
+                        g2p_finish<true, true>(ks, pts, i, s, record_pq);
                     } else {
                         spilled = true;
                     }
@@ -68,13 +74,15 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, Se
 
 // G2P of the spilled points, gathering node velocities from HBM.
 __global__ __launch_bounds__(kP2GThreads) void k_g2p_spill(KParams kp, PtsView pts, const double *__restrict__ grid,
-                                                           int record_pq, const int *spill, const int *spill_count) {
+                                                           int record_pq, const StepClock *clk, const int *spill,
+                                                           const int *spill_count, PtsView in) {
+    if (clk) record_pq = clk->record;
     const int count = *spill_count;
     const double c4h = 4. * kp.h_inv;
     for (int k = blockIdx.x * kP2GThreads + threadIdx.x; k < count; k += gridDim.x * kP2GThreads) {
         const int i = spill[k];
         PointState s;
-        load_g2p_inputs(pts, i, s);
+        load_g2p_state(in, i, s);
         const KParams ks = kp.ens_n > 0 ? scene_of_point(kp, s.pos) : kp;
         int base[3];
         double fx[3];
@@ -82,25 +90,36 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p_spill(KParams kp, PtsView p
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
         gather_global(ks, grid, base, w, d, c4h, s.vel, s.Bp);
-        g2p_finish<true>(ks, pts, i, s, record_pq);
+        g2p_finish<true, true>(ks, pts, i, s, record_pq);
     }
 }
 
 } // namespace
 
 hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid, int record_pq, int *err,
-                      int *lists, int n_blocks, hipStream_t stream) {
+                      int *lists, int n_blocks, const StepClock *clk, bool count_zeroed, hipStream_t stream,
+                      const PtsView *src) {
     if (n <= 0) return hipSuccess;
     int *spill_count = lists, *spill = lists + 1;
-    k_zero_count<<<1, 1, 0, stream>>>(spill_count);
+    if (!count_zeroed) k_zero_count<<<1, 1, 0, stream>>>(spill_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
-    if (kp.ens_n > 0)
-        k_g2p<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, err, spill, spill_count);
-    else k_g2p<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, err, spill, spill_count);
+    const PtsView in = src ? *src : pts;
+    if (kp.ens_n > 0 && src)
+        k_g2p<true, true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,
+                                                                spill_count, in);
+    else if (kp.ens_n > 0)
+        k_g2p<true, false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,
+                                                                 spill_count, in);
+    else if (src)
+        k_g2p<false, true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,
+                                                                 spill_count, in);
+    else
+        k_g2p<false, false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,
+                                                                  spill_count, in);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_g2p_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid, record_pq, spill, spill_count);
+    k_g2p_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid, record_pq, clk, spill, spill_count, in);
     return hipGetLastError();
 }
 

@@ -26,11 +26,15 @@ __global__ void k_zero_count(int *c) { *c = 0; }
 // LDS (two tiles and the records, about 79 KB) already limits the kernel to
 // two workgroups per CU; the bound keeps the compiler within the matching
 // 256 registers instead of spilling past them into AGPRs.
-template <bool kEns>
+// kOOP: the point state is read from src and the new state written to pts
+// (the calibration adjoint keeps every substep's state); otherwise both are pts.
+template <bool kEns, bool kOOP>
 __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
                                                        const double *__restrict__ grid_a, GridState gs_b, int cur_b,
-                                                       int record_pq, int store_vc, int *err, int *spill,
-                                                       int *spill_count) {
+                                                       int record_pq, const StepClock *clk, int store_vc, int *err,
+                                                       int *spill, int *spill_count, PtsView src) {
+    if (clk) record_pq = clk->record;
+    const PtsView in = kOOP ? src : pts;
     __shared__ P2GShared sh;
     __shared__ double sv[3][kTileAlloc];
     const int t = threadIdx.x;
@@ -41,7 +45,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
     // what G2P reads of the next chunk's points is loaded during the scatter
     PointState pf;
     pf.util = ICEMPM_STATUS_DISABLED;
-    if (c.valid() && c.c0 + t < c.end) load_g2p_state(pts, c.c0 + t, pf);
+    if (c.valid() && c.c0 + t < c.end) load_g2p_state(in, c.c0 + t, pf);
     while (c.valid()) {
         const KParams ks = scene_view<kEns>(kp, c.brick);
         int org[3], bc[3];
@@ -64,6 +68,8 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         s.Jp = pf.Jp;
         const ChunkCursor nx = cursor_next(c, seg);
         bool updated = false, spilled = false;
+        if (kOOP && (s.util & ICEMPM_STATUS_DISABLED) && i < c.end)
+            pts.at(ICEMPM_IDX_UTILITY, i) = bits_to_double(s.util);
         if (!(s.util & ICEMPM_STATUS_DISABLED)) {
             int base[3];
             double fx[3];
@@ -89,7 +95,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rcbrt(Je * Je), s.Bp, s.Jp, org, err);
         __syncthreads();
         pf.util = ICEMPM_STATUS_DISABLED;
-        if (nx.valid() && nx.c0 + t < nx.end) load_g2p_state(pts, nx.c0 + t, pf);
+        if (nx.valid() && nx.c0 + t < nx.end) load_g2p_state(in, nx.c0 + t, pf);
         p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
         if (c.last()) p2g_flush(sh, t, ks, org, bc, gs_b, bl);
         c = nx;
@@ -100,15 +106,17 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
 // of the new state straight into grid B.
 __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView pts,
                                                              const double *__restrict__ grid_a, GridState gs_b,
-                                                             int cur_b, int record_pq, int store_vc, int *err,
-                                                             const int *spill, const int *spill_count) {
+                                                             int cur_b, int record_pq, const StepClock *clk,
+                                                             int store_vc, int *err, const int *spill,
+                                                             const int *spill_count, PtsView in) {
+    if (clk) record_pq = clk->record;
     const int count = *spill_count;
     const double c4h = 4. * kp.h_inv;
     const BrickList bl = brick_list(gs_b, cur_b);
     for (int k = blockIdx.x * kP2GThreads + threadIdx.x; k < count; k += gridDim.x * kP2GThreads) {
         const int i = spill[k];
         PointState s;
-        load_g2p_inputs(pts, i, s);
+        load_g2p_state(in, i, s);
         const KParams ks = kp.ens_n > 0 ? scene_of_point(kp, s.pos) : kp;
         int base[3];
         double fx[3];
@@ -116,7 +124,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
         gather_global(ks, grid_a, base, w, d, c4h, s.vel, s.Bp);
-        g2p_finish<true>(ks, pts, i, s, record_pq, store_vc != 0);
+        g2p_finish<true, true>(ks, pts, i, s, record_pq, store_vc != 0);
         const int code = scatter_point_global(ks, gs_b, bl, s);
         if (code) raise_error(err, code);
     }
@@ -126,22 +134,29 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
 
 hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid_a, GridState gs_b,
                         int cur_b, int record_pq, int store_vc, int *err, int *lists, int n_blocks,
-                        hipStream_t stream) {
+                        const StepClock *clk, bool count_zeroed, hipStream_t stream, const PtsView *src) {
     if (n <= 0) return hipSuccess;
     int *spill_count = lists, *spill = lists + 1;
-    k_zero_count<<<1, 1, 0, stream>>>(spill_count);
+    if (!count_zeroed) k_zero_count<<<1, 1, 0, stream>>>(spill_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
-    if (kp.ens_n > 0)
-        k_g2p2g<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, store_vc,
-                                                            err, spill, spill_count);
+    const PtsView in = src ? *src : pts;
+    if (kp.ens_n > 0 && src)
+        k_g2p2g<true, true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,
+                                                                  store_vc, err, spill, spill_count, in);
+    else if (kp.ens_n > 0)
+        k_g2p2g<true, false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,
+                                                                   store_vc, err, spill, spill_count, in);
+    else if (src)
+        k_g2p2g<false, true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,
+                                                                   store_vc, err, spill, spill_count, in);
     else
-        k_g2p2g<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, store_vc,
-                                                             err, spill, spill_count);
+        k_g2p2g<false, false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq,
+                                                                    clk, store_vc, err, spill, spill_count, in);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_g2p2g_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid_a, gs_b, cur_b, record_pq, store_vc, err, spill,
-                                                    spill_count);
+    k_g2p2g_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid_a, gs_b, cur_b, record_pq, clk, store_vc, err,
+                                                    spill, spill_count, in);
     return hipGetLastError();
 }
 

@@ -14,7 +14,9 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / 64;
 
-__global__ __launch_bounds__(kThreads) void k_reset_grid(KParams kp, GridState gs, int cur) {
+// Zeroes the bricks of list `cur` and their flags, and the other list's count
+// (the list the next P2G appends to); one wave per brick, grid-stride.
+__device__ __forceinline__ void reset_bricks(const KParams &kp, const GridState &gs, int cur) {
     const int count = gs.brick_count[cur];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -29,6 +31,10 @@ __global__ __launch_bounds__(kThreads) void k_reset_grid(KParams kp, GridState g
     if (blockIdx.x == 0 && threadIdx.x == 0) gs.brick_count[cur ^ 1] = 0;
 }
 
+__global__ __launch_bounds__(kThreads) void k_reset_grid(KParams kp, GridState gs, int cur) {
+    reset_bricks(kp, gs, cur);
+}
+
 __global__ void k_clear_all(KParams kp, GridState gs, long long nbricks) {
     long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nbricks * kBrickNodes; i += stride) {
@@ -39,13 +45,26 @@ __global__ void k_clear_all(KParams kp, GridState gs, long long nbricks) {
     }
 }
 
+__global__ void k_clock_set(StepClock *clk, double t, long long step, int every) {
+    clk->t = t;
+    clk->step = step;
+    clk->every = every;
+    clk->record = 0;
+}
+
+// c0 + t * v rounded as two separate operations, as the host evaluates the
+// indenter law (model_3d.cpp:38); no fused multiply-add.
+__device__ __forceinline__ double law_center(double c0, double t, double v) {
+#pragma clang fp contract(off)
+    return c0 + t * v;
+}
+
 // partition_kernel_update_nodes (mpm_kernels.cuh:138-232) for the nodes of
-// the listed bricks; the indenter is the reference cylinder along z or a sphere.
-// One scene: the indenter center is (cx, cy, cz). Ensembles: each scene's
-// center is its c0 + t * velocity, walls and force bins are the scene's.
+// the bricks in list `cur`, one wave per brick, grid-stride; the indenter is
+// the reference cylinder along z or a sphere.
 template <bool kEns>
-__global__ __launch_bounds__(kThreads) void k_update_nodes(KParams kp0, GridState gs, int cur, double cx0,
-                                                           double cy0, double cz0, double t, double *bins0) {
+__device__ __forceinline__ void update_bricks(const KParams &kp0, const GridState &gs, int cur, double cx0, double cy0,
+                                              double cz0, double t, double *bins0) {
     const int count = gs.brick_count[cur];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -140,6 +159,33 @@ __global__ __launch_bounds__(kThreads) void k_update_nodes(KParams kp0, GridStat
     }
 }
 
+// One scene: the indenter center is (cx, cy, cz). Ensembles: each scene's
+// center is its c0 + t * velocity, walls and force bins are the scene's.
+// The work of UpdateExtras x follows in the same launch: the reset touches
+// another grid, and the clock slot written is not the one read.
+template <bool kEns>
+__global__ __launch_bounds__(kThreads) void k_update_nodes(KParams kp0, GridState gs, int cur, double cx0,
+                                                           double cy0, double cz0, double t, double *bins0,
+                                                           UpdateExtras x) {
+    if (x.clk_in) {
+        t = x.clk_in->t + kp0.dt;
+        cx0 = law_center(kp0.ind_c0[0], t, kp0.ind_vel[0]);
+        cy0 = law_center(kp0.ind_c0[1], t, kp0.ind_vel[1]);
+        cz0 = law_center(kp0.ind_c0[2], t, kp0.ind_vel[2]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            const long long step = x.clk_in->step;
+            const int every = x.clk_in->every;
+            x.clk_out->t = t;
+            x.clk_out->step = step + 1;
+            x.clk_out->every = every;
+            x.clk_out->record = (step % every) == 0 ? 1 : 0;
+        }
+    }
+    if (x.zero && blockIdx.x == 0 && threadIdx.x == 0) *x.zero = 0;
+    update_bricks<kEns>(kp0, gs, cur, cx0, cy0, cz0, t, bins0);
+    if (x.reset.grid) reset_bricks(kp0, x.reset, x.reset_cur);
+}
+
 __global__ void k_grid_to_xmajor(KParams kp, const double *grid, double *out) {
     const long long nn = (long long)kp.gx * kp.gy * kp.gz;
     long long stride = (long long)gridDim.x * blockDim.x;
@@ -166,11 +212,17 @@ hipError_t launch_clear_grid_all(const KParams &kp, GridState gs, long long nbri
 }
 
 hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double cx, double cy, double cz, double t,
-                               double *bins, hipStream_t stream) {
-    if (kp.ens_n > 0) k_update_nodes<true><<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, t, bins);
-    else k_update_nodes<false><<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, t, bins);
+                               double *bins, const UpdateExtras &x, hipStream_t stream) {
+    if (kp.ens_n > 0) k_update_nodes<true><<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, t, bins, x);
+    else k_update_nodes<false><<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, t, bins, x);
     return hipGetLastError();
 }
+
+hipError_t launch_clock_set(StepClock *clk, double t, long long step, int every, hipStream_t stream) {
+    k_clock_set<<<1, 1, 0, stream>>>(clk, t, step, every);
+    return hipGetLastError();
+}
+
 
 hipError_t launch_grid_to_xmajor(const KParams &kp, const double *grid, double *out, hipStream_t stream) {
     k_grid_to_xmajor<<<2048, 256, 0, stream>>>(kp, grid, out);

@@ -104,6 +104,21 @@ class BatchedEnsemble:
     def set_fusion(self, on: bool) -> None:
         check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))
 
+    def set_sort_period(self, period: int) -> None:
+        """Regroup the points by grid brick every `period` substeps (0: never)."""
+        check(lib().icempm_set_sort_period(self.ctx, int(period)))
+
+    def set_graph(self, on: bool) -> None:
+        """Replay each run of substeps between re-sorts from a captured hipGraph."""
+        check(lib().icempm_set_graph(self.ctx, 1 if on else 0))
+
+    def graph_captures(self) -> tuple[int, int]:
+        """(graphs captured so far, graphs kept)."""
+        n = C.c_longlong(0)
+        k = C.c_int(0)
+        check(lib().icempm_graph_captures(self.ctx, C.byref(n), C.byref(k)))
+        return n.value, k.value
+
     def autotune(self, trial_substeps: int = 10) -> dict:
         """icempm_autotune over all scenes together (the points are left as they are)."""
         t = capi.Tuning()

@@ -78,6 +78,17 @@ class GPUImplementation6:
     def set_fusion(self, on: bool) -> None:
         check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))
 
+    def set_graph(self, on: bool) -> None:
+        """Replay each run of substeps between re-sorts from a captured hipGraph."""
+        check(lib().icempm_set_graph(self.ctx, 1 if on else 0))
+
+    def graph_captures(self) -> tuple[int, int]:
+        """(graphs captured so far, graphs kept)."""
+        n = C.c_longlong(0)
+        k = C.c_int(0)
+        check(lib().icempm_graph_captures(self.ctx, C.byref(n), C.byref(k)))
+        return n.value, k.value
+
     def set_blocks_per_cu(self, blocks_per_cu: int) -> None:
         check(lib().icempm_set_blocks_per_cu(self.ctx, int(blocks_per_cu)))
 

@@ -71,7 +71,8 @@ typedef struct icempm_Indenter {
 #define ICEMPM_PHASE_UPDATE 3
 #define ICEMPM_PHASE_G2P 4
 #define ICEMPM_PHASE_G2P2G 5 /* fused G2P + next P2G (icempm_set_fusion) */
-#define ICEMPM_N_PHASES 6
+#define ICEMPM_PHASE_GRAPH 6 /* one replay of a captured run of substeps (icempm_set_graph) */
+#define ICEMPM_N_PHASES 7
 
 typedef struct icempm_ctx icempm_ctx;
 
@@ -149,6 +150,19 @@ int icempm_step(icempm_ctx *ctx, int n_substeps, double *simulation_time, int si
  * issues the four phases per substep like the reference. Results agree to
  * rounding (summation order). */
 int icempm_set_fusion(icempm_ctx *ctx, int on);
+
+/* Extension (no reference counterpart): with graph mode on, icempm_step
+ * captures each run of substeps between two brick re-sorts (the re-sort
+ * included) once into a hipGraph and replays it; the simulation time and the
+ * p/q record flag then come from a device-side clock that the replays
+ * advance, so one graph serves every run of the same length that starts from
+ * the same buffer state (at most 16 are kept; a change of parameters, point
+ * count or launch settings captures anew). Results are those of the launched
+ * path. Turning it off destroys the graphs. With timing on, each replay is
+ * timed as ICEMPM_PHASE_GRAPH (per-phase times are then not recorded). */
+int icempm_set_graph(icempm_ctx *ctx, int on);
+/* Graphs captured so far and graphs currently kept. */
+int icempm_graph_captures(icempm_ctx *ctx, long long *captures_out, int *cached_out);
 
 /* GPU_Implementation6::synchronize (gpu_implementation6.cpp:513-520) */
 int icempm_synchronize(icempm_ctx *ctx);
