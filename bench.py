@@ -14,6 +14,10 @@ The same workload at 27 and 64 points per cell (134x67x66 and 100x50x50
 cells on the same grid, 200 timed steps each) is reported as
 "points_per_cell".
 
+The 2M-point block with the damage model (256x128x128 grid, dt = 5e-6 s,
+2000 substeps, the indenter force read back every 10 substeps) is reported
+as "damage".
+
 The calibration adjoint (reverse mode of the elastic step on the
 200,000-point calibration scene: 40x25x25 cells on a 128x64x64 grid, 400
 substeps, force sampled every 10, checkpoints every 20) is timed after it and
@@ -216,6 +220,50 @@ def run_ppc(args, name: str) -> dict:
     return out
 
 
+def run_damage(args) -> dict:
+    """The 2M-point block with the damage model (256x128x128 grid, dt 5e-6,
+    2000 substeps, the indenter force read back every 10 substeps):
+    particle-updates/s including the read-backs, the force curve's peak and
+    end, and the damage field at the end."""
+    from icempm_3dm_amd import GPUImplementation6, capi, scenes
+
+    sc = scenes.BLOCK_2M
+    hs = scenes.make_points(sc)
+    n = hs.size
+    g = GPUImplementation6(0)
+    g.hssoa = hs
+    prms = scenes.make_params(sc)
+    g.allocate_arrays(prms, hs.capacity)
+    g.update_constants(prms)
+    g.set_indenter(*sc.indenter())
+    g.set_damage(capi.Damage.make(**scenes.BLOCK_2M_DAMAGE))
+    g.set_sort_period(50)
+    g.set_blocks_per_cu(16)
+    g.transfer_ponts_to_device()
+    forces = []
+    t_sim = 0.0
+    g.synchronize()
+    t0 = time.perf_counter()
+    for k in range(sc.steps // 10):
+        g.reset_indenter_force_accumulator()
+        t_sim = g.step(10, t_sim, 10 * k)
+        forces.append(g.read_indenter_bins().reshape(-1, 3).sum(0) / 10)
+    elapsed = time.perf_counter() - t0
+    err = g.device_error_code()
+    g.transfer_from_device()
+    D = capi.damage_of(g.hssoa.utility()[:n])
+    crushed = int(np.count_nonzero(g.hssoa.utility()[:n] & capi.STATUS_CRUSHED))
+    g.close()
+    fmag = np.linalg.norm(np.array(forces), axis=1)
+    return {"workload": f"{sc.name}: {n} points on a 256x128x128 grid, dt={sc.dt:g}, {sc.steps} substeps, "
+                        f"damage (softening strain {scenes.BLOCK_2M_DAMAGE['softening_strain']}, residual cohesion "
+                        f"{scenes.BLOCK_2M_DAMAGE['residual_cohesion']}), force read back every 10 substeps",
+            "value": n * sc.steps / elapsed, "ms_per_step": elapsed * 1e3 / sc.steps, "device_error": err,
+            "force_peak_N": float(fmag.max()), "force_end_N": float(fmag[-1]), "crushed_points": crushed,
+            "damaged_points": int(np.count_nonzero(D > 0)), "damage_max": float(D.max()),
+            "fully_damaged_points": int(np.count_nonzero(D >= 1.0))}
+
+
 def run_adjoint(args) -> dict:
     """Forward with checkpoints and backward of the calibration run, and the
     plain step on the same scene; times per substep."""
@@ -275,11 +323,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble", type=int, default=64, help="scenes of the ensemble run; 0: skip it")
     ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint timing")
+    ap.add_argument("--damage", type=int, default=1, help="0: skip the 2M-point damage run")
     ap.add_argument("--ppc-sweep", type=int, default=1, help="0: skip the 27 and 64 points-per-cell runs")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: launch settings from icempm_autotune (else --fuse / --sort-period / --blocks-per-cu)")
     ap.add_argument("--blocks-per-cu", type=int, default=8)
-    ap.add_argument("--only", choices=("ensemble", "adjoint"), default=None,
+    ap.add_argument("--only", choices=("ensemble", "adjoint", "damage"), default=None,
                     help="run only this part and print its result (not the bench line)")
     args = ap.parse_args()
 
@@ -297,6 +346,10 @@ def main() -> int:
     if args.only == "ensemble":
         log(f"bench: ensemble of {args.ensemble} scenes: {args.warmup} warm-up + {args.steps} timed steps")
         print(json.dumps({"ensemble": run_ensemble(args)}), flush=True)
+        return 0
+    if args.only == "damage":
+        log("bench: 2M-point block with damage (2000 substeps)")
+        print(json.dumps({"damage": run_damage(args)}), flush=True)
         return 0
     if args.only == "adjoint":
         log("bench: calibration adjoint (200k points, 400 substeps)")
@@ -366,6 +419,11 @@ def main() -> int:
             log(f"bench: {name}")
             ppc[name] = run_ppc(args, name)
 
+    damage = None
+    if args.damage:
+        log("bench: 2M-point block with damage (2000 substeps)")
+        damage = run_damage(args)
+
     adjoint = None
     if args.adjoint:
         log("bench: calibration adjoint (200k points, 400 substeps)")
@@ -416,6 +474,7 @@ def main() -> int:
         "points_per_cell": ppc,
         "ensemble": ensemble,
         "adjoint": adjoint,
+        "damage": damage,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

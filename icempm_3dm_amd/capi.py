@@ -126,6 +126,31 @@ class Indenter(C.Structure):
         return ind
 
 
+class Damage(C.Structure):
+    """icempm_Damage: softening damage of the return mapping (D in bits 32-63
+    of the utility word as an IEEE single)."""
+
+    _fields_ = [("enabled", C.c_int), ("softening_strain", C.c_double), ("residual_cohesion", C.c_double)]
+
+    @staticmethod
+    def make(softening_strain: float, residual_cohesion: float = 0.1, enabled: bool = True) -> "Damage":
+        return Damage(1 if enabled else 0, float(softening_strain), float(residual_cohesion))
+
+
+def damage_of(util_bits: np.ndarray) -> np.ndarray:
+    """D of int64 utility words: the single in bits 32-63, read into [0, 1] (NaN as 0)."""
+    d = (np.asarray(util_bits).astype(np.int64).view(np.uint64) >> np.uint64(32)).astype(np.uint32).view(np.float32)
+    d = np.where(np.isnan(d), 0.0, d).astype(np.float64)
+    return np.clip(d, 0.0, 1.0)
+
+
+def with_damage(util_bits: np.ndarray, D) -> np.ndarray:
+    """util_bits with D (rounded to single) in bits 32-63."""
+    u = np.asarray(util_bits).astype(np.int64).view(np.uint64)
+    hi = np.asarray(np.broadcast_to(np.float32(D), u.shape)).astype(np.float32).view(np.uint32).astype(np.uint64)
+    return ((u & np.uint64(0xFFFFFFFF)) | (hi << np.uint64(32))).view(np.int64)
+
+
 class AdjointRun(C.Structure):
     """icempm_AdjointRun: substeps, force sample length, checkpoint period, start time."""
 
@@ -171,9 +196,11 @@ _PROTOS = {
     "icempm_allocate": (C.c_int, [_ctxp, _pp, C.c_int]),
     "icempm_update_constants": (C.c_int, [_ctxp, _pp]),
     "icempm_set_indenter": (C.c_int, [_ctxp, C.POINTER(Indenter)]),
+    "icempm_set_damage": (C.c_int, [_ctxp, C.POINTER(Damage)]),
     "icempm_transfer_points_to_device": (C.c_int, [_ctxp, _dp, C.c_uint, C.c_uint, C.c_int]),
     "icempm_transfer_from_device": (C.c_int, [_ctxp, _dp, C.c_uint, C.c_uint, _ip, _dp]),
     "icempm_reset_indenter_force_accumulator": (C.c_int, [_ctxp]),
+    "icempm_read_indenter_bins": (C.c_int, [_ctxp, C.POINTER(C.c_double)]),
     "icempm_reset_grid": (C.c_int, [_ctxp]),
     "icempm_p2g": (C.c_int, [_ctxp]),
     "icempm_update_nodes": (C.c_int, [_ctxp, C.c_double, C.c_double, C.c_double]),

@@ -78,6 +78,7 @@ struct icempm_ctx {
     bool have_params = false;
     icempm_Indenter ind{};
     bool ind_custom = false;
+    icempm_Damage damage{0, 1.0, 1.0};
     KParams kp{};
 
     // particles
@@ -253,8 +254,8 @@ icempm_Indenter reference_indenter(const icempm_SimParams3D &P) {
 
 // Kernel parameters of one scene whose node (0,0,0) sits at node `origin` of a
 // stored grid of nb bricks per axis.
-KParams scene_kparams(const icempm_SimParams3D &P, const icempm_Indenter &ind, const int origin[3],
-                      const int nb[3]) {
+KParams scene_kparams(const icempm_SimParams3D &P, const icempm_Indenter &ind, const icempm_Damage &dmg,
+                      const int origin[3], const int nb[3]) {
     KParams k{};
     k.gx = P.GridXTotal;
     k.gy = P.GridY;
@@ -289,6 +290,9 @@ KParams scene_kparams(const icempm_SimParams3D &P, const icempm_Indenter &ind, c
     k.rm_cbrt_je_new = std::cbrt(k.rm_je_new);
     k.drag = P.DragGlobal;
     k.setup_type = P.SetupType;
+    k.dmg_on = dmg.enabled ? 1 : 0;
+    k.dmg_rate = 1.0 / dmg.softening_strain;
+    k.dmg_soft = 1.0 - dmg.residual_cohesion;
     k.ind_subdiv = P.IndenterSubdivisions;
     k.ind_rsq = P.IndRSq;
     k.ind_shape = ind.shape;
@@ -306,7 +310,7 @@ void fill_kparams(icempm_ctx *ctx) {
     int nb[3] = {(P.GridXTotal + 3) / 4, (P.GridY + 3) / 4, (P.GridZ + 3) / 4};
     if (ctx->ens.n > 0)
         for (int d = 0; d < 3; d++) nb[d] = ctx->ens.slot_b[d] * ctx->ens.nslot[d];
-    ctx->kp = scene_kparams(P, ctx->ind, origin, nb);
+    ctx->kp = scene_kparams(P, ctx->ind, ctx->damage, origin, nb);
     if (ctx->ens.n > 0) {
         ctx->kp.ens_n = ctx->ens.n;
         for (int d = 0; d < 3; d++) {
@@ -748,7 +752,7 @@ int ens_upload_slots(icempm_ctx *ctx) {
     std::vector<KParams> table((size_t)nslots);
     for (int k = 0; k < nslots; k++) {
         const int sc = k < E.n ? k : 0;
-        KParams e = scene_kparams(E.prms[sc], E.ind[sc], &E.origin[3 * sc], nb);
+        KParams e = scene_kparams(E.prms[sc], E.ind[sc], ctx->damage, &E.origin[3 * sc], nb);
         e.bins_off = E.bins_off[sc];
         e.ens_n = E.n;
         for (int d = 0; d < 3; d++) {
@@ -867,6 +871,21 @@ int icempm_set_indenter(icempm_ctx *ctx, const icempm_Indenter *ind) {
     return ICEMPM_OK;
 }
 
+int icempm_set_damage(icempm_ctx *ctx, const icempm_Damage *damage) {
+    if (!ctx || !damage) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (damage->enabled && !(damage->softening_strain > 0))
+        return fail(ICEMPM_ERR_INVALID_ARG, "softening_strain must be positive");
+    if (damage->enabled && !(damage->residual_cohesion >= 0 && damage->residual_cohesion <= 1))
+        return fail(ICEMPM_ERR_INVALID_ARG, "residual_cohesion must be in [0, 1]");
+    ctx->damage = damage->enabled ? *damage : icempm_Damage{0, 1.0, 1.0};
+    fill_kparams(ctx);
+    if (ctx->ens.n > 0 && ctx->ens.d_slots) {
+        ICEMPM_TRY(set_device(ctx));
+        ICEMPM_TRY(ens_upload_slots(ctx));
+    }
+    return ICEMPM_OK;
+}
+
 namespace {
 int upload_points(icempm_ctx *ctx, const double *hssoa, unsigned capacity, unsigned offset, int n_points);
 }
@@ -971,6 +990,16 @@ int icempm_transfer_from_device(icempm_ctx *ctx, double *hssoa, unsigned capacit
 int icempm_reset_indenter_force_accumulator(icempm_ctx *ctx) {
     ICEMPM_TRY(check_ready(ctx));
     return zero_bins(ctx);
+}
+
+int icempm_read_indenter_bins(icempm_ctx *ctx, double *indenter_bins) {
+    ICEMPM_TRY(check_ready(ctx));
+    if (!indenter_bins) return fail(ICEMPM_ERR_INVALID_ARG, "null output");
+    if (ctx->nbins > 0)
+        ICEMPM_HIP(hipMemcpyAsync(indenter_bins, ctx->bins, sizeof(double) * ctx->nbins, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
 }
 
 int icempm_reset_grid(icempm_ctx *ctx) {
@@ -1426,6 +1455,8 @@ int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double
     if (ctx->ens.n > 0) return fail(ICEMPM_ERR_INVALID_ARG, "the adjoint runs one scene, not an ensemble");
     if (ctx->prms.SetupType == 2) return fail(ICEMPM_ERR_INVALID_ARG, "the adjoint does not cover SetupType 2");
     if (ctx->nbins <= 0) return fail(ICEMPM_ERR_INVALID_ARG, "the force samples need IndenterSubdivisions > 0");
+    if (ctx->damage.enabled)
+        return fail(ICEMPM_ADJ_UNSUPPORTED, "the adjoint does not differentiate the damage model");
     const icempm_AdjointRun R = *run;
     if (R.n_steps <= 0 || R.sample_every <= 0 || R.checkpoint_every <= 0 || R.n_steps % R.sample_every ||
         R.n_steps % R.checkpoint_every)

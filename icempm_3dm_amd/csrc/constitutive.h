@@ -205,6 +205,19 @@ __device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3]
 
 // ---------------------------------------------------------------- plasticity
 // Per-particle state of the return mapping (Point3D, point_3d.h:14-27).
+// Damage D of a point: the IEEE single in bits 32-63 of its utility word,
+// read into [0, 1] (NaN as 0).
+__device__ __forceinline__ double damage_of(long long util) {
+    const float d = __uint_as_float((unsigned)((unsigned long long)util >> 32));
+    return d >= 1.0f ? 1.0 : (d > 0.0f ? (double)d : 0.0);
+}
+
+// util with D (rounded to single) in bits 32-63.
+__device__ __forceinline__ long long with_damage(long long util, double D) {
+    const unsigned bits = __float_as_uint((float)D);
+    return (long long)(((unsigned long long)util & 0xffffffffull) | ((unsigned long long)bits << 32));
+}
+
 struct PState {
     double Fe[9];
     double Jp_inv;
@@ -275,21 +288,39 @@ __device__ __forceinline__ void usvt(const double U[9], const double s[3], const
 // crushed point after compute_pq. J^{2/3} is taken as J^2 * (J^{-2/3})^2 and
 // the tension-branch constants come precomputed; the rest keeps the
 // reference's expression order.
+// kDamage (the kernels instantiate it where kp.dmg_on is set): the cohesion
+// of the tension cutoff and of the Drucker-Prager line is scaled by
+// f = 1 - dmg_soft * D, and D grows by the equivalent plastic shear strain
+// over the softening strain, stored as soon as the branch has fixed the new
+// shear stress. Without it the code is the reference's return mapping alone.
+template <bool kDamage = false>
 __device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
     double U[9], V[9], sig[3], sig2[3];
     svd3(p.Fe, U, sig, sig2, V);
+    constexpr bool damage = kDamage;
+    double pt = kp.dp_threshold_p;
+    if constexpr (damage) pt *= 1.0 - kp.dmg_soft * damage_of(p.util);
     double snew[3];
     bool project = true;
-    if (p.p_tr < -kp.dp_threshold_p || p.Jp_inv < 1) {
-        snew[0] = snew[1] = snew[2] = kp.rm_cbrt_je_new;
-        p.Jp_inv *= kp.rm_je_new / p.Je_tr;
+    if (p.p_tr < -pt || p.Jp_inv < 1) {
+        if (!damage || pt == kp.dp_threshold_p) {
+            snew[0] = snew[1] = snew[2] = kp.rm_cbrt_je_new;
+            p.Jp_inv *= kp.rm_je_new / p.Je_tr;
+        } else {
+            const double je_new = sqrt(-2. * (-pt) / kp.kappa + 1.);
+            snew[0] = snew[1] = snew[2] = cbrt(je_new);
+            p.Jp_inv *= je_new / p.Je_tr;
+        }
+        // the deviatoric stress drops from q_tr to 0
+        if constexpr (damage)
+            p.util = with_damage(p.util, fmin(1.0, damage_of(p.util) + p.q_tr / (3.0 * kp.mu) * kp.dmg_rate));
     } else {
         const double pmax = kp.ice_comp;
         double q_n_1;
         if (p.p_tr > pmax) {
             q_n_1 = 0;
         } else {
-            const double q_from_dp = (p.p_tr + kp.dp_threshold_p) * kp.tan_phi;
+            const double q_from_dp = (p.p_tr + pt) * kp.tan_phi;
             const double pmin = -kp.ice_tens;
             const double q_from_failure_surface =
                 2 * sqrt((pmax - p.p_tr) * (p.p_tr - pmin)) * kp.ice_shear / (pmax - pmin);
@@ -297,6 +328,10 @@ __device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
         }
         project = p.q_tr >= q_n_1;
         if (project) {
+            // the deviatoric stress drops from q_tr to q_n_1
+            if constexpr (damage)
+                p.util = with_damage(p.util,
+                                     fmin(1.0, damage_of(p.util) + (p.q_tr - q_n_1) / (3.0 * kp.mu) * kp.dmg_rate));
             const double Je_tr_sq = p.Je_tr * p.Je_tr;
             const double s = kp.mu * p.rc_Jsq;
             const double mean = (sig2[0] + sig2[1] + sig2[2]) / 3;

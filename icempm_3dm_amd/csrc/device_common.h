@@ -35,6 +35,9 @@ struct KParams {
     double rm_je_new, rm_cbrt_je_new;
     double drag;
     int setup_type;
+    // damage model: on/off, 1 / softening strain, 1 - residual cohesion
+    int dmg_on;
+    double dmg_rate, dmg_soft;
 
     int ind_shape, ind_subdiv;
     double ind_rsq;

@@ -192,12 +192,13 @@ __device__ __forceinline__ void gather_global(const KParams &kp, const double *g
 // Jp_inv, then advection, F update, p/q, crush test; stores the 26 (+2 with
 // record_pq) lines and leaves the new state in s. Without store_vc the
 // velocity and C lines are not stored: a caller whose next reader of the
-// point is another G2P (which recomputes both from the grid) saves the 96 B. With kInlineReturnMap the
-// return mapping of a crushed point runs here and false is returned;
-// otherwise the trial F is stored and true is returned for a crushed solid
-// point, whose return mapping (return_map_point) must follow before its F
-// is used. With kPreloaded, s.Fe and s.Jp already hold the stored values.
-template <bool kInlineReturnMap, bool kPreloaded = false>
+// point is another G2P (which recomputes both from the grid) saves the
+// 96 B. With kInlineReturnMap the return mapping of a crushed point runs
+// here and false is returned; otherwise the trial F is stored and true is
+// returned for a crushed solid point, whose return mapping must follow
+// before its F is used. With kPreloaded, s.Fe and s.Jp already hold the
+// stored values. kDamage: the return mapping with the damage model.
+template <bool kInlineReturnMap, bool kPreloaded = false, bool kDamage = false>
 __device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i, PointState &s, int record_pq,
                                            bool store_vc = true) {
     const double dt = kp.dt;
@@ -237,7 +238,7 @@ __device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i
         if (!(p.util & ICEMPM_STATUS_CRUSHED)) check_failure_surface(kp, p);
         crushed = (p.util & ICEMPM_STATUS_CRUSHED) != 0;
         if (kInlineReturnMap && crushed) {
-            return_map(kp, p);
+            return_map<kDamage>(kp, p);
             crushed = false;
         }
     }
@@ -257,21 +258,14 @@ __device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i
     return crushed;
 }
 
-// ComputePQ + ComputeSVD + Wolper_Drucker_Prager (mpm_kernels.cuh:477-544)
-// of point i from the trial F and Jp_inv that g2p_finish stored; writes the
-// projected F and Jp_inv back.
-__device__ __forceinline__ void return_map_point(const KParams &kp, PtsView pts, int i) {
-    PState p;
-#pragma unroll
-    for (int k = 0; k < 9; k++) p.Fe[k] = pts.at(ICEMPM_FE00 + k, i);
-    p.Jp_inv = pts.at(ICEMPM_IDX_JP_INV, i);
-    p.util = 0;
-    compute_pq(kp, p);
-    return_map(kp, p);
-#pragma unroll
-    for (int k = 0; k < 9; k++) pts.at(ICEMPM_FE00 + k, i) = p.Fe[k];
-    pts.at(ICEMPM_IDX_JP_INV, i) = p.Jp_inv;
+// g2p_finish with the return mapping inline and the state preloaded, the
+// damage model chosen at run time (kernels off the hot path).
+__device__ __forceinline__ void g2p_finish_any(const KParams &kp, PtsView pts, int i, PointState &s, int record_pq,
+                                               bool store_vc = true) {
+    if (kp.dmg_on) g2p_finish<true, true, true>(kp, pts, i, s, record_pq, store_vc);
+    else g2p_finish<true, true, false>(kp, pts, i, s, record_pq, store_vc);
 }
+
 
 // Workgroup-local list of the chunk's crushed points (built after a barrier
 // from per-lane flags, no global atomics).

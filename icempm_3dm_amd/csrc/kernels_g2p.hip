@@ -18,8 +18,8 @@ __global__ void k_zero_count(int *c) { *c = 0; }
 constexpr int kG2PWaves = kP2GThreads / 64;
 
 // kOOP: the point state is read from `in` and the new state written to pts;
-// otherwise in is pts.
-template <bool kEns, bool kOOP>
+// otherwise in is pts. kDamage: the return mapping with the damage model.
+template <bool kEns, bool kOOP, bool kDamage>
 __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, SegInfo seg,
                                                      const double *__restrict__ grid, int record_pq,
                                                      const StepClock *clk, int *err, int *spill, int *spill_count,
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, Se
                         double w3[3][3], d[3][3];
                         stencil_weights(fx, w3, d);
                         gather_tile(sv, tile_index(rx, ry, rz), w3, d, c4h, s.vel, s.Bp);
-                        g2p_finish<true, true>(ks, pts, i, s, record_pq);
+                        g2p_finish<true, true, kDamage>(ks, pts, i, s, record_pq);
                     } else {
                         spilled = true;
                     }
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p_spill(KParams kp, PtsView p
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
         gather_global(ks, grid, base, w, d, c4h, s.vel, s.Bp);
-        g2p_finish<true, true>(ks, pts, i, s, record_pq);
+        g2p_finish_any(ks, pts, i, s, record_pq);
     }
 }
 
@@ -106,18 +106,23 @@ hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, SegInfo seg, const 
     if (e != hipSuccess) return e;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
     const PtsView in = src ? *src : pts;
-    if (kp.ens_n > 0 && src)
-        k_g2p<true, true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,
-                                                                spill_count, in);
-    else if (kp.ens_n > 0)
-        k_g2p<true, false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,
-                                                                 spill_count, in);
-    else if (src)
-        k_g2p<false, true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,
-                                                                 spill_count, in);
-    else
-        k_g2p<false, false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,
-                                                                  spill_count, in);
+#define ICEMPM_G2P(E, O, D)                                                                                      \
+    k_g2p<E, O, D><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,           \
+                                                         spill_count, in)
+    // the adjoint's out-of-place passes never run with the damage model
+    const bool ens = kp.ens_n > 0, dmg = kp.dmg_on != 0;
+    if (src && dmg) return hipErrorInvalidValue;
+    if (src) {
+        if (ens) ICEMPM_G2P(true, true, false);
+        else ICEMPM_G2P(false, true, false);
+    } else if (dmg) {
+        if (ens) ICEMPM_G2P(true, false, true);
+        else ICEMPM_G2P(false, false, true);
+    } else {
+        if (ens) ICEMPM_G2P(true, false, false);
+        else ICEMPM_G2P(false, false, false);
+    }
+#undef ICEMPM_G2P
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_g2p_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid, record_pq, clk, spill, spill_count, in);
     return hipGetLastError();

@@ -28,7 +28,8 @@ __global__ void k_zero_count(int *c) { *c = 0; }
 // 256 registers instead of spilling past them into AGPRs.
 // kOOP: the point state is read from src and the new state written to pts
 // (the calibration adjoint keeps every substep's state); otherwise both are pts.
-template <bool kEns, bool kOOP>
+// kDamage: the return mapping with the damage model (kp.dmg_on).
+template <bool kEns, bool kOOP, bool kDamage>
 __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
                                                        const double *__restrict__ grid_a, GridState gs_b, int cur_b,
                                                        int record_pq, const StepClock *clk, int store_vc, int *err,
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
                     double w[3][3], d[3][3];
                     stencil_weights(fx, w, d);
                     gather_tile(sv, tile_index(rx, ry, rz), w, d, c4h, s.vel, s.Bp);
-                    g2p_finish<true, true>(ks, pts, i, s, record_pq, store_vc != 0);
+                    g2p_finish<true, true, kDamage>(ks, pts, i, s, record_pq, store_vc != 0);
                     updated = true;
                 } else {
                     spilled = true;
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
         double w[3][3], d[3][3];
         stencil_weights(fx, w, d);
         gather_global(ks, grid_a, base, w, d, c4h, s.vel, s.Bp);
-        g2p_finish<true, true>(ks, pts, i, s, record_pq, store_vc != 0);
+        g2p_finish_any(ks, pts, i, s, record_pq, store_vc != 0);
         const int code = scatter_point_global(ks, gs_b, bl, s);
         if (code) raise_error(err, code);
     }
@@ -142,18 +143,23 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
     if (e != hipSuccess) return e;
     n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
     const PtsView in = src ? *src : pts;
-    if (kp.ens_n > 0 && src)
-        k_g2p2g<true, true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,
-                                                                  store_vc, err, spill, spill_count, in);
-    else if (kp.ens_n > 0)
-        k_g2p2g<true, false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,
-                                                                   store_vc, err, spill, spill_count, in);
-    else if (src)
-        k_g2p2g<false, true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,
-                                                                   store_vc, err, spill, spill_count, in);
-    else
-        k_g2p2g<false, false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq,
-                                                                    clk, store_vc, err, spill, spill_count, in);
+#define ICEMPM_G2P2G(E, O, D)                                                                                    \
+    k_g2p2g<E, O, D><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,       \
+                                                           store_vc, err, spill, spill_count, in)
+    // the adjoint's out-of-place passes never run with the damage model
+    const bool ens = kp.ens_n > 0, dmg = kp.dmg_on != 0;
+    if (src && dmg) return hipErrorInvalidValue;
+    if (src) {
+        if (ens) ICEMPM_G2P2G(true, true, false);
+        else ICEMPM_G2P2G(false, true, false);
+    } else if (dmg) {
+        if (ens) ICEMPM_G2P2G(true, false, true);
+        else ICEMPM_G2P2G(false, false, true);
+    } else {
+        if (ens) ICEMPM_G2P2G(true, false, false);
+        else ICEMPM_G2P2G(false, false, false);
+    }
+#undef ICEMPM_G2P2G
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_g2p2g_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid_a, gs_b, cur_b, record_pq, clk, store_vc, err,
                                                     spill, spill_count, in);

@@ -108,6 +108,11 @@ class BatchedEnsemble:
         """Regroup the points by grid brick every `period` substeps (0: never)."""
         check(lib().icempm_set_sort_period(self.ctx, int(period)))
 
+    def set_damage(self, damage: "capi.Damage | None") -> None:
+        """The damage model of the return mapping (None: off)."""
+        d = damage if damage is not None else capi.Damage(0, 1.0, 1.0)
+        check(lib().icempm_set_damage(self.ctx, C.byref(d)))
+
     def set_graph(self, on: bool) -> None:
         """Replay each run of substeps between re-sorts from a captured hipGraph."""
         check(lib().icempm_set_graph(self.ctx, 1 if on else 0))

@@ -78,6 +78,11 @@ class GPUImplementation6:
     def set_fusion(self, on: bool) -> None:
         check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))
 
+    def set_damage(self, damage: "capi.Damage | None") -> None:
+        """The damage model of the return mapping (None: off)."""
+        d = damage if damage is not None else capi.Damage(0, 1.0, 1.0)
+        check(lib().icempm_set_damage(self.ctx, C.byref(d)))
+
     def set_graph(self, on: bool) -> None:
         """Replay each run of substeps between re-sorts from a captured hipGraph."""
         check(lib().icempm_set_graph(self.ctx, 1 if on else 0))
@@ -167,11 +172,8 @@ class GPUImplementation6:
 
     def read_indenter_bins(self) -> np.ndarray:
         """Raw accumulators without moving points (synchronizes)."""
-        n = C.c_int(0)
         raw = np.zeros(self.prms.IndenterArraySize())
-        tmp = HostSideSOA(self.hssoa.capacity)
-        check(lib().icempm_transfer_from_device(self.ctx, dptr(tmp.host_buffer), tmp.capacity, 0, C.byref(n),
-                                                dptr(raw)))
+        check(lib().icempm_read_indenter_bins(self.ctx, dptr(raw)))
         return raw
 
     def download_grid(self) -> np.ndarray:

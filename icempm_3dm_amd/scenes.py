@@ -74,6 +74,9 @@ class BlockScene:
 
 BLOCK_100K = BlockScene("block_100k", (64, 32, 32), (25, 20, 25), steps=200)
 BLOCK_2M = BlockScene("block_2m", (256, 128, 128), (100, 50, 50), dt=5e-6, steps=2000)
+# the damage model of the 2M-point block run: full damage at 1% equivalent
+# plastic shear strain, softening to 10% of the cohesion
+BLOCK_2M_DAMAGE = {"softening_strain": 0.01, "residual_cohesion": 0.1}
 BLOCK_16M = BlockScene("block_16m", (512, 256, 256), (200, 100, 100), steps=500)
 BLOCK_16M_PPC27 = replace(BLOCK_16M, name="block_16m_ppc27", cells=(134, 67, 66), ppc_axis=3)
 BLOCK_16M_PPC64 = replace(BLOCK_16M, name="block_16m_ppc64", cells=(100, 50, 50), ppc_axis=4)

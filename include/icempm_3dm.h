@@ -47,7 +47,8 @@ extern "C" {
  * The indicator keeps the largest code raised since the last
  * update_constants / clear. Offending particles are skipped, not written. */
 #define ICEMPM_DEVERR_NONFINITE 76
-/* Adjoint runs only: 91 a liquid point (no adjoint of the water pressure). */
+/* Adjoint runs only: 91 a liquid point (no adjoint of the water pressure) or
+ * the damage model enabled. */
 #define ICEMPM_ADJ_UNSUPPORTED 91
 
 /* ---- indenter law (extension) -----------------------------------------
@@ -63,6 +64,28 @@ typedef struct icempm_Indenter {
     double center_initial[3];
     double velocity[3];
 } icempm_Indenter;
+
+/* Extension (no reference counterpart): softening damage in the return
+ * mapping of crushed solid points. Each point carries a damage D in [0, 1],
+ * stored as an IEEE single in bits 32-63 of its utility word (bits the
+ * reference never sets and setPartition keeps, proxypoint3d.cpp:89-110; an
+ * all-zero upper word is D = 0, and a value that is not a number in [0, 1]
+ * reads as the nearest of 0 and 1, NaN as 0). With damage enabled the
+ * cohesion of the Drucker-Prager surface is scaled by
+ * f = 1 - (1 - residual_cohesion) * D: the tension cutoff pressure becomes
+ * f * DP_threshold_p (also in the tension branch's Je_new) and the
+ * Drucker-Prager line (p + f * DP_threshold_p) * tan(phi). After the return
+ * mapping, D += (q_tr - q_new) / (3 mu) / softening_strain, capped at 1,
+ * where q_tr is the trial equivalent shear stress and q_new the one after the
+ * return (q_n+1 for a projection, 0 for the tension branch, q_tr when
+ * inside): D grows with the equivalent plastic shear strain and reaches 1 at
+ * softening_strain. The yield surface of a substep uses the damage from
+ * before it. */
+typedef struct icempm_Damage {
+    int enabled;
+    double softening_strain;  /* > 0 */
+    double residual_cohesion; /* in [0, 1]: cohesion fraction left at D = 1 */
+} icempm_Damage;
 
 /* Timed phases reported by icempm_read_timings (ms accumulated). */
 #define ICEMPM_PHASE_SORT 0
@@ -106,6 +129,13 @@ int icempm_update_constants(icempm_ctx *ctx, const icempm_SimParams3D *prms);
 /* Extension: replace the indenter law (shape, initial center, velocity). */
 int icempm_set_indenter(icempm_ctx *ctx, const icempm_Indenter *ind);
 
+/* Extension: the damage model (icempm_Damage; disabled by default), for the
+ * single scene or every scene of an ensemble. ICEMPM_ERR_INVALID_ARG for a
+ * softening strain that is not positive or a residual cohesion outside
+ * [0, 1]. The calibration adjoint does not differentiate it:
+ * icempm_adjoint_forward returns ICEMPM_ADJ_UNSUPPORTED while it is enabled. */
+int icempm_set_damage(icempm_ctx *ctx, const icempm_Damage *damage);
+
 /* GPU_Partition_3D::transfer_points_from_soa_to_device
  * (gpu_partition_3d.cu:201-215): n_points points from host lines
  * [offset, offset+n_points) become the device's points, in that order. */
@@ -125,6 +155,10 @@ int icempm_transfer_from_device(icempm_ctx *ctx, double *hssoa, unsigned capacit
 
 /* GPU_Partition_3D::reset_indenter_force_accumulator (gpu_partition_3d.cu:145-150) */
 int icempm_reset_indenter_force_accumulator(icempm_ctx *ctx);
+/* Extension: the indenter force accumulators alone (IndenterArraySize()
+ * doubles; for an ensemble all scenes' concatenated in scene order), without
+ * the point copy of icempm_transfer_from_device. Synchronizes. */
+int icempm_read_indenter_bins(icempm_ctx *ctx, double *indenter_bins);
 /* GPU_Partition_3D::reset_grid (gpu_partition_3d.cu:222-229) */
 int icempm_reset_grid(icempm_ctx *ctx);
 /* GPU_Partition_3D::p2g (gpu_partition_3d.cu:231-241) */

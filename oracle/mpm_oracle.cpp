@@ -295,14 +295,42 @@ Mat3 usvt(const Mat3 &U, const double s[3], const Mat3 &V) {
     return mul_abt(US, V);
 }
 
-// Wolper_Drucker_Prager (mpm_kernels.cuh:494-544)
+// The damage model (no reference counterpart; process-wide, set by
+// oracle_set_damage): disabled unless set.
+icempm_Damage g_damage = {0, 1.0, 1.0};
+
+// Damage of a point: the IEEE single in bits 32-63 of the utility word,
+// read into [0, 1] (NaN as 0).
+double damage_of(long long util) {
+    const uint32_t bits = (uint32_t)((unsigned long long)util >> 32);
+    float d;
+    std::memcpy(&d, &bits, sizeof(d));
+    return d >= 1.0f ? 1.0 : (d > 0.0f ? (double)d : 0.0);
+}
+
+long long with_damage(long long util, double D) {
+    const float d = (float)D;
+    uint32_t bits;
+    std::memcpy(&bits, &d, sizeof(bits));
+    return (long long)(((unsigned long long)util & 0xffffffffull) | ((unsigned long long)bits << 32));
+}
+
+// Wolper_Drucker_Prager (mpm_kernels.cuh:494-544). With the damage model,
+// the cohesion terms are scaled by f = 1 - (1 - residual) D and D grows by
+// (q_tr - q_new) / (3 mu) / softening_strain afterwards.
 void wolper_drucker_prager(const icempm_SimParams3D &P, Point &p) {
     const double mu = P.mu;
     const double kappa = P.kappa;
     const double tan_phi = P.DP_tan_phi;
-    const double DP_threshold_p = P.DP_threshold_p;
+    double DP_threshold_p = P.DP_threshold_p;
     const double pmax = P.IceCompressiveStrength;
     const double qmax = P.IceShearStrength;
+    double D = 0.0;
+    if (g_damage.enabled) {
+        D = damage_of(p.utility_data);
+        DP_threshold_p *= 1.0 - (1.0 - g_damage.residual_cohesion) * D;
+    }
+    double q_new = p.q_tr;
 
     if (p.p_tr < -DP_threshold_p || p.Jp_inv < 1) {
         double p_new = -DP_threshold_p;
@@ -311,6 +339,7 @@ void wolper_drucker_prager(const icempm_SimParams3D &P, Point &p) {
         double s[3] = {cbrt_Je_new, cbrt_Je_new, cbrt_Je_new};
         p.Fe = usvt(p.U, s, p.V);
         p.Jp_inv *= Je_new / p.Je_tr;
+        q_new = 0.0;
     } else {
         double q_n_1;
         if (p.p_tr > pmax) {
@@ -339,7 +368,12 @@ void wolper_drucker_prager(const icempm_SimParams3D &P, Point &p) {
             double snew[3];
             for (int i = 0; i < 3; i++) snew[i] = std::sqrt(coef * nv[i] + mean);
             p.Fe = usvt(p.U, snew, p.V);
+            q_new = q_n_1;
         }
+    }
+    if (g_damage.enabled) {
+        const double dD = (p.q_tr - q_new) / (3.0 * mu) / g_damage.softening_strain;
+        p.utility_data = with_damage(p.utility_data, std::fmin(1.0, D + dD));
     }
 }
 
@@ -430,6 +464,10 @@ int p2g_particle(const icempm_SimParams3D &P, const double *soa, int pitch, int 
 } // namespace
 
 extern "C" {
+
+void oracle_set_damage(const icempm_Damage *damage) {
+    g_damage = damage ? *damage : icempm_Damage{0, 1.0, 1.0};
+}
 
 void oracle_weights(const icempm_SimParams3D *prms, const double pos[3], int base[3], double fx[3],
                     double w[9]) {
@@ -1007,6 +1045,7 @@ extern "C" {
 
 int oracle_adj_forward(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
                        int n, int n_steps, int sample_every, double *forces) {
+    if (g_damage.enabled) return ICEMPM_ADJ_UNSUPPORTED; // the damage model is not differentiated
     const icempm_SimParams3D &P = *prms;
     const size_t nn = (size_t)P.GridXTotal * P.GridY * P.GridZ;
     std::vector<double> soa(soa0, soa0 + (size_t)pitch * ICEMPM_N_PTS_ARRAYS);
@@ -1046,6 +1085,7 @@ int oracle_adj_forward(const icempm_SimParams3D *prms, const icempm_Indenter *in
 
 int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
                         int n, int n_steps, int sample_every, const double *dL_dF, double *grad_mu_kappa) {
+    if (g_damage.enabled) return ICEMPM_ADJ_UNSUPPORTED; // the damage model is not differentiated
     // grad_mu_kappa: dL/d(mu, kappa, DP_threshold_p, DP_tan_phi)
     const icempm_SimParams3D &P = *prms;
     const size_t nn = (size_t)P.GridXTotal * P.GridY * P.GridZ;

@@ -44,6 +44,10 @@ void oracle_stable_sort(const uint32_t *keys, int n, int32_t *perm);
  * compacts enabled points (stable), then sorts by reference key. Returns new size. */
 int oracle_remove_disabled_and_sort(const icempm_SimParams3D *prms, double *soa, int pitch, int n);
 
+/* The damage model of the return mapping (no reference counterpart; see
+ * icempm_Damage). Process-wide; NULL disables it. */
+void oracle_set_damage(const icempm_Damage *damage);
+
 /* Quadratic B-spline base node, fractional offset and weights (mpm_kernels.cuh:65-81). */
 void oracle_weights(const icempm_SimParams3D *prms, const double pos[3], int base[3], double fx[3],
                     double w[9] /* w[3*i + axis] */);

@@ -40,6 +40,7 @@ _PROTOS = {
     "oracle_step": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, C.c_int]),
     "oracle_adj_forward": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "oracle_adj_gradient": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]),
+    "oracle_set_damage": (None, [C.POINTER(capi.Damage)]),
 }
 
 ORACLE_LIB_NOFMA = ROOT / "oracle" / "lib" / "libmpm_oracle_nofma.so"
@@ -72,6 +73,12 @@ def lib_nofma() -> C.CDLL:
     if _LIB_NOFMA is None:
         _LIB_NOFMA = _load(ORACLE_LIB_NOFMA)
     return _LIB_NOFMA
+
+
+def set_damage(damage) -> None:
+    """The damage model of both oracle builds (None: off)."""
+    for h in (lib(), lib_nofma()):
+        h.oracle_set_damage(C.byref(damage) if damage is not None else None)
 
 
 def _d(a):
