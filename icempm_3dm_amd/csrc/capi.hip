@@ -921,6 +921,29 @@ int icempm_ensemble_transfer_points(icempm_ctx *ctx, const double *hssoa, unsign
                                                             std::to_string(i));
         }
     }
+    // Each scene's node offset inside its slot: the slot is whole bricks and
+    // leaves at least 4 spare nodes per axis, so the scene box can move by
+    // up to 3 nodes and still own every brick it touches. The offset puts the
+    // lowest base node of the scene's points on a brick boundary, which
+    // minimises the bricks (and so the segments) the block occupies.
+    for (int i = 0; i < E.n; i++) {
+        const int sl[3] = {i / (E.nslot[1] * E.nslot[2]), (i / E.nslot[2]) % E.nslot[1], i % E.nslot[2]};
+        const int g[3] = {E.prms[i].GridXTotal, E.prms[i].GridY, E.prms[i].GridZ};
+        for (int d = 0; d < 3; d++) {
+            const double *line = hssoa + (size_t)(ICEMPM_POSX + d) * capacity;
+            long long lo = 0;
+            bool any = false;
+            for (int k = scene_first[i]; k < scene_first[i + 1]; k++) {
+                const long long b = (long long)std::floor(line[k] * E.prms[i].cellsize_inv - 0.5);
+                if (!any || b < lo) lo = b;
+                any = true;
+            }
+            int shift = any ? (int)(((-lo) % 4 + 4) % 4) : 0;
+            if (shift > E.slot_b[d] * 4 - g[d]) shift = 0;
+            E.origin[3 * i + d] = sl[d] * E.slot_b[d] * 4 + shift;
+        }
+    }
+    ICEMPM_TRY(ens_upload_slots(ctx));
     // scene-local positions -> stored-grid positions
     std::vector<double> tmp((size_t)std::max(n, 1) * ICEMPM_N_PTS_ARRAYS);
     for (int l = 0; l < ICEMPM_N_PTS_ARRAYS; l++)

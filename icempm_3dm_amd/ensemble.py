@@ -79,6 +79,13 @@ class BatchedEnsemble:
         for i, (shape, center, vel) in enumerate(indenters):
             ind = capi.Indenter.make(shape, center, vel)
             check(lib().icempm_ensemble_set_indenter(self.ctx, i, C.byref(ind)))
+        self.n_bins = sum(p.IndenterArraySize() for p in self.params)
+        self.simulation_time = 0.0
+        self.simulation_step = 0
+        self.transfer_points_to_device()
+
+    def _read_layout(self) -> None:
+        """Each scene's node offset in the stored grid and first accumulator."""
         self.bins_offset = []
         self.origin = []
         for i in range(self.n_scenes):
@@ -87,10 +94,6 @@ class BatchedEnsemble:
             check(lib().icempm_ensemble_layout(self.ctx, i, o, C.byref(off)))
             self.origin.append(tuple(o))
             self.bins_offset.append(off.value)
-        self.n_bins = sum(p.IndenterArraySize() for p in self.params)
-        self.simulation_time = 0.0
-        self.simulation_step = 0
-        self.transfer_points_to_device()
 
     @property
     def n_points(self) -> int:
@@ -100,6 +103,7 @@ class BatchedEnsemble:
         first = (C.c_int * (self.n_scenes + 1))(*[int(x) for x in self.first])
         check(lib().icempm_ensemble_transfer_points(self.ctx, dptr(self.hssoa.host_buffer), self.hssoa.capacity,
                                                     first))
+        self._read_layout()  # the upload places each scene in its slot
 
     def set_fusion(self, on: bool) -> None:
         check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))

@@ -316,7 +316,10 @@ int icempm_allocate_ensemble(icempm_ctx *ctx, const icempm_SimParams3D *scenes, 
 int icempm_ensemble_set_indenter(icempm_ctx *ctx, int scene, const icempm_Indenter *ind);
 /* Upload the points of every scene: scene i owns host points
  * [scene_first[i], scene_first[i+1]) (scene_first has n_scenes+1 entries,
- * scene_first[0] = 0), positions in scene coordinates. */
+ * scene_first[0] = 0), positions in scene coordinates. Each scene's node
+ * offset inside its slot is chosen here (up to 3 nodes per axis, so that its
+ * points' lowest base node starts a brick); icempm_ensemble_layout reports
+ * it afterwards. */
 int icempm_ensemble_transfer_points(icempm_ctx *ctx, const double *hssoa, unsigned capacity,
                                     const int *scene_first);
 /* Where scene `scene` lives: node offset of its node (0,0,0) in the stored
