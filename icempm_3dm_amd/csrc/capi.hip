@@ -1230,7 +1230,7 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     icempm_Tuning top{keep.fuse, keep.sort_period, keep.blocks_per_cu, 1e300};
     rc = ICEMPM_OK;
     for (int fuse = 1; fuse >= 0 && rc == ICEMPM_OK; fuse--)
-        for (int period : {10, 20, 50})
+        for (int period : {20, 50, 100, 200})
             for (int bpc : {4, 8, 16}) {
                 if (rc != ICEMPM_OK) break;
                 ctx->fuse = fuse != 0;
@@ -1242,16 +1242,20 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
                 double t = 0.0;
                 if (rc == ICEMPM_OK) rc = icempm_step(ctx, 2, &t, 0);
                 if (rc != ICEMPM_OK) break;
+                // whole periods, each opening with its re-sort: the timed cost
+                // is the sort amortised over the period plus its substeps
+                const int len = (trial_substeps + period - 1) / period * period;
+                ctx->substeps_since_sort = period;
                 float ms = 0.f;
                 if (hipEventRecord(ev0, ctx->stream) != hipSuccess) rc = fail(ICEMPM_ERR_HIP, "hipEventRecord");
-                if (rc == ICEMPM_OK) rc = icempm_step(ctx, trial_substeps, &t, 2);
+                if (rc == ICEMPM_OK) rc = icempm_step(ctx, len, &t, 2);
                 if (rc == ICEMPM_OK && hipEventRecord(ev1, ctx->stream) != hipSuccess)
                     rc = fail(ICEMPM_ERR_HIP, "hipEventRecord");
                 if (rc == ICEMPM_OK && (hipEventSynchronize(ev1) != hipSuccess ||
                                         hipEventElapsedTime(&ms, ev0, ev1) != hipSuccess))
                     rc = fail(ICEMPM_ERR_HIP, "autotune timing");
-                if (rc == ICEMPM_OK && ms / trial_substeps < top.ms_per_substep)
-                    top = icempm_Tuning{fuse, period, bpc, (double)ms / trial_substeps};
+                if (rc == ICEMPM_OK && ms / len < top.ms_per_substep)
+                    top = icempm_Tuning{fuse, period, bpc, (double)ms / len};
             }
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
