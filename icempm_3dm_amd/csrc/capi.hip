@@ -80,6 +80,7 @@ struct icempm_ctx {
     bool ind_custom = false;
     icempm_Damage damage{0, 1.0, 1.0};
     KParams kp{};
+    double *rmc = nullptr; // kp.rmc: the return mapping's constants (device)
 
     // particles
     bool allocated = false;
@@ -131,6 +132,7 @@ struct icempm_ctx {
         std::vector<long long> bins_off;
         std::vector<int> first; // n + 1: each scene's points in upload order
         KParams *d_slots = nullptr;
+        double *d_rmc = nullptr; // kRmcSize return-mapping constants per slot
     } ens;
 
     // calibration adjoint: checkpoints of a forward run, replayed per segment
@@ -303,7 +305,20 @@ KParams scene_kparams(const icempm_SimParams3D &P, const icempm_Indenter &ind, c
     return k;
 }
 
-void fill_kparams(icempm_ctx *ctx) {
+// The return mapping's constants of a parameter block (KParams::rmc layout).
+void rmc_values(const KParams &k, double out[kRmcSize]) {
+    out[kRmcThresholdP] = k.dp_threshold_p;
+    out[kRmcTanPhi] = k.tan_phi;
+    out[kRmcJeNew] = k.rm_je_new;
+    out[kRmcCbrtJeNew] = k.rm_cbrt_je_new;
+    out[kRmcDmgRate] = k.dmg_rate;
+    out[kRmcDmgSoft] = k.dmg_soft;
+}
+
+// ctx->kp from the context's parameters, indenter and damage model; once the
+// device block exists, its return-mapping constants are written in stream
+// order (after the work already queued, which keeps the old ones).
+int fill_kparams(icempm_ctx *ctx) {
     const icempm_SimParams3D &P = ctx->prms;
     if (!ctx->ind_custom) ctx->ind = reference_indenter(P);
     const int origin[3] = {0, 0, 0};
@@ -319,6 +334,15 @@ void fill_kparams(icempm_ctx *ctx) {
         }
         ctx->kp.ens_slots = ctx->ens.d_slots;
     }
+    ctx->kp.rmc = ctx->rmc;
+    if (ctx->rmc) {
+        double v[kRmcSize];
+        rmc_values(ctx->kp, v);
+        ICEMPM_HIP(hipSetDevice(ctx->device));
+        ICEMPM_HIP(hipMemcpyAsync(ctx->rmc, v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
+        ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return ICEMPM_OK;
 }
 
 template <typename T> int dev_alloc(T **p, size_t count, const char *what) {
@@ -365,6 +389,10 @@ void free_all(icempm_ctx *ctx) {
     }
     f(ctx->ens.d_slots);
     ctx->ens.d_slots = nullptr;
+    f(ctx->ens.d_rmc);
+    ctx->ens.d_rmc = nullptr;
+    f(ctx->rmc);
+    ctx->rmc = nullptr;
     {
         auto &A = ctx->adj;
         // theta lives inside the forces allocation
@@ -680,7 +708,8 @@ int check_grid(const icempm_SimParams3D *prms) {
 // ctx->ens must already describe the scenes of an ensemble (or be empty).
 int alloc_common(icempm_ctx *ctx, const icempm_SimParams3D &prms, int n_points_capacity, long long nbins) {
     ctx->prms = prms;
-    fill_kparams(ctx);
+    ICEMPM_TRY(dev_alloc(&ctx->rmc, kRmcSize, "return-mapping constants"));
+    ICEMPM_TRY(fill_kparams(ctx));
     const long long nb = (long long)ctx->kp.nbx * ctx->kp.nby * ctx->kp.nbz;
     if (nb * kBrickNodes > (1ll << 40) || nb > 0x7FFFFFF)
         return fail(ICEMPM_ERR_INVALID_ARG, "grid too large");
@@ -750,9 +779,12 @@ int ens_upload_slots(icempm_ctx *ctx) {
     const int nslots = E.nslot[0] * E.nslot[1] * E.nslot[2];
     const int nb[3] = {E.slot_b[0] * E.nslot[0], E.slot_b[1] * E.nslot[1], E.slot_b[2] * E.nslot[2]};
     std::vector<KParams> table((size_t)nslots);
+    std::vector<double> rmc((size_t)nslots * kRmcSize);
     for (int k = 0; k < nslots; k++) {
         const int sc = k < E.n ? k : 0;
         KParams e = scene_kparams(E.prms[sc], E.ind[sc], ctx->damage, &E.origin[3 * sc], nb);
+        rmc_values(e, &rmc[(size_t)k * kRmcSize]);
+        e.rmc = E.d_rmc + (size_t)k * kRmcSize;
         e.bins_off = E.bins_off[sc];
         e.ens_n = E.n;
         for (int d = 0; d < 3; d++) {
@@ -764,6 +796,7 @@ int ens_upload_slots(icempm_ctx *ctx) {
     }
     ICEMPM_HIP(hipMemcpyAsync(E.d_slots, table.data(), sizeof(KParams) * nslots, hipMemcpyHostToDevice,
                               ctx->stream));
+    ICEMPM_HIP(hipMemcpyAsync(E.d_rmc, rmc.data(), sizeof(double) * rmc.size(), hipMemcpyHostToDevice, ctx->stream));
     ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
     return ICEMPM_OK;
 }
@@ -818,8 +851,10 @@ int icempm_allocate_ensemble(icempm_ctx *ctx, const icempm_SimParams3D *scenes, 
     stored.DomainDimensionX = stored.GridXTotal * stored.cellsize;
     ICEMPM_TRY(alloc_common(ctx, stored, n_points_capacity, nbins));
     ICEMPM_TRY(dev_alloc(&ctx->ens.d_slots, (size_t)E.nslot[0] * E.nslot[1] * E.nslot[2], "ensemble slots"));
+    ICEMPM_TRY(dev_alloc(&ctx->ens.d_rmc, (size_t)E.nslot[0] * E.nslot[1] * E.nslot[2] * kRmcSize,
+                         "ensemble return-mapping constants"));
     ICEMPM_TRY(ens_upload_slots(ctx));
-    fill_kparams(ctx);
+    ICEMPM_TRY(fill_kparams(ctx));
     ctx->have_params = true;
     return clear_err(ctx);
 }
@@ -855,7 +890,7 @@ int icempm_update_constants(icempm_ctx *ctx, const icempm_SimParams3D *prms) {
         (long long)prms->IndenterSubdivisions * prms->GridZ * 3 != ctx->nbins)
         return fail(ICEMPM_ERR_INVALID_ARG, "grid or indenter sizes differ from the allocation");
     ctx->prms = *prms;
-    fill_kparams(ctx);
+    ICEMPM_TRY(fill_kparams(ctx));
     ctx->have_params = true;
     return clear_err(ctx);
 }
@@ -867,8 +902,7 @@ int icempm_set_indenter(icempm_ctx *ctx, const icempm_Indenter *ind) {
         return fail(ICEMPM_ERR_INVALID_ARG, "unknown indenter shape");
     ctx->ind = *ind;
     ctx->ind_custom = true;
-    fill_kparams(ctx);
-    return ICEMPM_OK;
+    return fill_kparams(ctx);
 }
 
 int icempm_set_damage(icempm_ctx *ctx, const icempm_Damage *damage) {
@@ -878,7 +912,7 @@ int icempm_set_damage(icempm_ctx *ctx, const icempm_Damage *damage) {
     if (damage->enabled && !(damage->residual_cohesion >= 0 && damage->residual_cohesion <= 1))
         return fail(ICEMPM_ERR_INVALID_ARG, "residual_cohesion must be in [0, 1]");
     ctx->damage = damage->enabled ? *damage : icempm_Damage{0, 1.0, 1.0};
-    fill_kparams(ctx);
+    ICEMPM_TRY(fill_kparams(ctx));
     if (ctx->ens.n > 0 && ctx->ens.d_slots) {
         ICEMPM_TRY(set_device(ctx));
         ICEMPM_TRY(ens_upload_slots(ctx));

@@ -298,14 +298,16 @@ __device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
     double U[9], V[9], sig[3], sig2[3];
     svd3(p.Fe, U, sig, sig2, V);
     constexpr bool damage = kDamage;
-    double pt = kp.dp_threshold_p;
-    if constexpr (damage) pt *= 1.0 - kp.dmg_soft * damage_of(p.util);
+    const double *rmc = kp.rmc;
+    const double pt0 = rmc[kRmcThresholdP];
+    double pt = pt0;
+    if constexpr (damage) pt *= 1.0 - rmc[kRmcDmgSoft] * damage_of(p.util);
     double snew[3];
     bool project = true;
     if (p.p_tr < -pt || p.Jp_inv < 1) {
-        if (!damage || pt == kp.dp_threshold_p) {
-            snew[0] = snew[1] = snew[2] = kp.rm_cbrt_je_new;
-            p.Jp_inv *= kp.rm_je_new / p.Je_tr;
+        if (!damage || pt == pt0) {
+            snew[0] = snew[1] = snew[2] = rmc[kRmcCbrtJeNew];
+            p.Jp_inv *= rmc[kRmcJeNew] / p.Je_tr;
         } else {
             const double je_new = sqrt(-2. * (-pt) / kp.kappa + 1.);
             snew[0] = snew[1] = snew[2] = cbrt(je_new);
@@ -313,14 +315,15 @@ __device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
         }
         // the deviatoric stress drops from q_tr to 0
         if constexpr (damage)
-            p.util = with_damage(p.util, fmin(1.0, damage_of(p.util) + p.q_tr / (3.0 * kp.mu) * kp.dmg_rate));
+            p.util = with_damage(p.util,
+                                 fmin(1.0, damage_of(p.util) + p.q_tr / (3.0 * kp.mu) * rmc[kRmcDmgRate]));
     } else {
         const double pmax = kp.ice_comp;
         double q_n_1;
         if (p.p_tr > pmax) {
             q_n_1 = 0;
         } else {
-            const double q_from_dp = (p.p_tr + pt) * kp.tan_phi;
+            const double q_from_dp = (p.p_tr + pt) * rmc[kRmcTanPhi];
             const double pmin = -kp.ice_tens;
             const double q_from_failure_surface =
                 2 * sqrt((pmax - p.p_tr) * (p.p_tr - pmin)) * kp.ice_shear / (pmax - pmin);
@@ -330,8 +333,8 @@ __device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
         if (project) {
             // the deviatoric stress drops from q_tr to q_n_1
             if constexpr (damage)
-                p.util = with_damage(p.util,
-                                     fmin(1.0, damage_of(p.util) + (p.q_tr - q_n_1) / (3.0 * kp.mu) * kp.dmg_rate));
+                p.util = with_damage(
+                    p.util, fmin(1.0, damage_of(p.util) + (p.q_tr - q_n_1) / (3.0 * kp.mu) * rmc[kRmcDmgRate]));
             const double Je_tr_sq = p.Je_tr * p.Je_tr;
             const double s = kp.mu * p.rc_Jsq;
             const double mean = (sig2[0] + sig2[1] + sig2[2]) / 3;

@@ -38,6 +38,11 @@ struct KParams {
     // damage model: on/off, 1 / softening strain, 1 - residual cohesion
     int dmg_on;
     double dmg_rate, dmg_soft;
+    // The return mapping's constants in device memory (kRmc* below). Only
+    // crushed points read them, through this pointer at the point of use:
+    // as kernel arguments they would be held in scalar registers for the
+    // whole fused kernel, which then spills.
+    const double *rmc;
 
     int ind_shape, ind_subdiv;
     double ind_rsq;
@@ -54,6 +59,10 @@ struct KParams {
     int ens_nslot[3];  // slots per axis
     const KParams *ens_slots;
 };
+
+// Layout of KParams::rmc.
+constexpr int kRmcThresholdP = 0, kRmcTanPhi = 1, kRmcJeNew = 2, kRmcCbrtJeNew = 3, kRmcDmgRate = 4, kRmcDmgSoft = 5;
+constexpr int kRmcSize = 6;
 
 // Parameter block of the scene that owns grid brick b (ensembles).
 __device__ __forceinline__ KParams scene_of_brick(const KParams &kp, long long b) {
