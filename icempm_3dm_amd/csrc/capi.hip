@@ -1264,7 +1264,7 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     icempm_Tuning top{keep.fuse, keep.sort_period, keep.blocks_per_cu, 1e300};
     rc = ICEMPM_OK;
     for (int fuse = 1; fuse >= 0 && rc == ICEMPM_OK; fuse--)
-        for (int period : {20, 50, 100, 200})
+        for (int period : {10, 20, 50, 100, 200})
             for (int bpc : {4, 8, 16}) {
                 if (rc != ICEMPM_OK) break;
                 ctx->fuse = fuse != 0;
