@@ -49,6 +49,9 @@ public:
         throw_if(icempm_update_constants(ctx_, &prms));
     }
     void set_indenter(const icempm_Indenter &ind) { throw_if(icempm_set_indenter(ctx_, &ind)); }
+    // extensions: damage model, hipGraph replay of the substep loop
+    void set_damage(const icempm_Damage &dmg) { throw_if(icempm_set_damage(ctx_, &dmg)); }
+    void set_graph(bool on) { throw_if(icempm_set_graph(ctx_, on ? 1 : 0)); }
     // gpu_implementation6.cpp:404-414
     void transfer_ponts_to_device(const double *host_buffer, unsigned capacity, unsigned size) {
         throw_if(icempm_transfer_points_to_device(ctx_, host_buffer, capacity, 0, (int)size));
@@ -87,6 +90,11 @@ public:
     }
     // gpu_implementation6.cpp:513-520
     void synchronize() { throw_if(icempm_synchronize(ctx_)); }
+    // extension: the indenter accumulators alone (no point copy); synchronizes
+    void read_indenter_bins(std::vector<double> &bins) {
+        bins.assign((size_t)prms_.IndenterSubdivisions * prms_.GridZ * 3, 0.0);
+        throw_if(icempm_read_indenter_bins(ctx_, bins.data()));
+    }
     // HostSideSOA::RemoveDisabledAndSort (host_side_soa.cpp:56-67), on the device
     int remove_disabled_and_sort() {
         int n = 0;
