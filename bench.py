@@ -179,9 +179,11 @@ def run_ensemble(args) -> dict:
     # launched substep by substep, then replayed from captured hipGraphs
     # (one graph per run between re-sorts, captured during the warm-up)
     elapsed_launch = timed(args.steps)
-    ens.set_graph(True)
-    elapsed = timed(args.steps)
-    captures = ens.graph_captures()[0]
+    elapsed, captures = elapsed_launch, 0
+    if args.graph:
+        ens.set_graph(True)
+        elapsed = timed(args.steps)
+        captures = ens.graph_captures()[0]
     import ctypes as C
     from icempm_3dm_amd import capi
     c = C.c_int(0)
@@ -190,7 +192,8 @@ def run_ensemble(args) -> dict:
     counts = [s.n_points for s in sw]
     ens.close()
     return {"scenes": len(sw), "points": n, "points_min": min(counts), "points_max": max(counts),
-            "value": n * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps, "mode": "hipGraph replay",
+            "value": n * args.steps / elapsed, "ms_per_step": elapsed * 1e3 / args.steps,
+            "mode": "hipGraph replay" if args.graph else "launched",
             "graph_captures": captures, "launched_value": n * args.steps / elapsed_launch,
             "launched_ms_per_step": elapsed_launch * 1e3 / args.steps, "device_error": code,
             "autotuned": tuning}
@@ -290,20 +293,23 @@ def run_adjoint(args) -> dict:
     gpu.synchronize()
     ts = time.perf_counter() - t0
     # the same substeps replayed from captured hipGraphs
-    gpu.set_graph(True)
-    t = gpu.step(50, t, 50 + n_steps)
-    gpu.synchronize()
-    t0 = time.perf_counter()
-    gpu.step(n_steps, t, 100 + n_steps)
-    gpu.synchronize()
-    tg = time.perf_counter() - t0
+    tg = None
+    if args.graph:
+        gpu.set_graph(True)
+        t = gpu.step(50, t, 50 + n_steps)
+        gpu.synchronize()
+        t0 = time.perf_counter()
+        gpu.step(n_steps, t, 100 + n_steps)
+        gpu.synchronize()
+        tg = time.perf_counter() - t0
     n = m.gpu.hssoa.size
     m.close()
     return {"workload": f"{CALIBRATION_SCENE.name}: {n} points on a 128x64x64 grid, {n_steps} substeps, dt=2e-6, "
                         f"force every {S}, checkpoints every {K}",
             "ratio": (tb / n_steps) / (ts / n_steps),
             "backward_ms_per_step": tb * 1e3 / n_steps, "forward_ms_per_step": tf * 1e3 / n_steps,
-            "step_ms_per_step": ts * 1e3 / n_steps, "step_graph_ms_per_step": tg * 1e3 / n_steps,
+            "step_ms_per_step": ts * 1e3 / n_steps,
+            "step_graph_ms_per_step": tg * 1e3 / n_steps if tg is not None else None,
             "dL_dE_ones": float(g[0]), "force_last": [float(x) for x in F[-1]]}
 
 
@@ -324,6 +330,8 @@ def main() -> int:
     ap.add_argument("--ensemble", type=int, default=64, help="scenes of the ensemble run; 0: skip it")
     ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint timing")
     ap.add_argument("--damage", type=int, default=1, help="0: skip the 2M-point damage run")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="0: skip the hipGraph replays (rocprofv3's kernel trace does not follow stream capture)")
     ap.add_argument("--ppc-sweep", type=int, default=1, help="0: skip the 27 and 64 points-per-cell runs")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: launch settings from icempm_autotune (else --fuse / --sort-period / --blocks-per-cu)")

@@ -1474,8 +1474,18 @@ int adj_run(icempm_ctx *ctx, int s0, int count, Slot slot, bool keep, bool recor
         if (keep)
             ICEMPM_HIP(hipMemcpyAsync(A.G + (size_t)4 * gp * j, ctx->grids[ga].grid, sizeof(double) * 4 * gp,
                                       hipMemcpyDeviceToDevice, ctx->stream));
+        // the update's launch also resets grid gb for the fused pass and
+        // clears the spill count
+        UpdateExtras x;
+        const bool fused = j + 1 < count;
+        if (fused) {
+            x.reset = ctx->gstate(gb);
+            x.reset_cur = ctx->grids[gb].cur;
+        }
+        x.zero = ctx->work;
         ICEMPM_HIP(launch_update_nodes(ctx->kp, ctx->gstate(ga), ctx->grids[ga].cur, c[0], c[1], c[2], A.times[s],
-                                       ctx->bins, UpdateExtras{}, ctx->stream));
+                                       ctx->bins, x, ctx->stream));
+        if (fused) ctx->grids[gb].cur ^= 1;
         if (record && (s + 1) % S == 0)
             ICEMPM_HIP(launch_sum_bins(ctx->bins, ctx->nbins, 1.0 / S, A.forces + 3 * (s / S), ctx->stream));
         if (keep)
@@ -1483,14 +1493,13 @@ int adj_run(icempm_ctx *ctx, int s0, int count, Slot slot, bool keep, bool recor
                                       hipMemcpyDeviceToDevice, ctx->stream));
         // G2P reads slot(j) and writes the new state to slot(j + 1)
         const PtsView in = adj_view(slot(j), A.n), out = adj_view(slot(j + 1), A.n);
-        if (j + 1 < count) {
-            ICEMPM_TRY(do_reset_grid(ctx, gb));
+        if (fused) {
             ICEMPM_HIP(launch_g2p2g(ctx->kp, out, A.n, seg, ctx->grids[ga].grid, ctx->gstate(gb), ctx->grids[gb].cur, 0,
-                                    1, ctx->err, ctx->work, ctx->p2g_blocks, nullptr, false, ctx->stream, &in));
+                                    1, ctx->err, ctx->work, ctx->p2g_blocks, nullptr, true, ctx->stream, &in));
             std::swap(ga, gb);
         } else {
             ICEMPM_HIP(launch_g2p(ctx->kp, out, A.n, seg, ctx->grids[ga].grid, 0, ctx->err, ctx->work,
-                                  ctx->g2p_blocks, nullptr, false, ctx->stream, &in));
+                                  ctx->g2p_blocks, nullptr, true, ctx->stream, &in));
         }
     }
     return ICEMPM_OK;
@@ -1628,6 +1637,8 @@ int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double gr
         ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(g), ctx->nbricks, ctx->stream));
         ctx->grids[g].cur = 0;
     }
+    // v_bar starts at zero; the reverse grid update leaves it zeroed after each substep
+    ICEMPM_HIP(hipMemsetAsync(A.vbar, 0, sizeof(double) * 4 * gp, ctx->stream));
     for (int seg_i = A.nseg - 1; seg_i >= 0; seg_i--) {
         const int s0 = seg_i * K;
         ICEMPM_HIP(hipMemcpyAsync(A.traj, A.ckpt + st * seg_i, sizeof(double) * st, hipMemcpyDeviceToDevice,
@@ -1639,7 +1650,6 @@ int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double gr
             for (int i = 0; i < 3; i++) fbar[i] = dL_dforces[3 * (s / S) + i] / S;
             const PtsView s_0 = adj_view(A.traj + st * j, n), s_1 = adj_view(A.traj + st * (j + 1), n);
             const double *G = A.G + (size_t)4 * gp * j, *V = A.V + (size_t)4 * gp * j;
-            ICEMPM_HIP(hipMemsetAsync(A.vbar, 0, sizeof(double) * 4 * gp, ctx->stream));
             ICEMPM_HIP(launch_adj_g2p_bwd(ctx->kp, s_0, s_1, bar, part, n, seg, V, gb, 0, A.theta, A.any_crushed,
                                           ctx->p2g_blocks, ctx->stream));
             ICEMPM_HIP(launch_adj_update_bwd(ctx->kp, G, A.vbar, A.Gbar, &A.centers[3 * s], fbar, ctx->stream));

@@ -119,7 +119,8 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
 hipError_t launch_adj_g2p_bwd(const KParams &kp, PtsView s0, PtsView s1, PtsView bar, PtsView part, int n,
                               SegInfo seg, const double *V, GridState gb, int cur_b, double *theta, bool plastic,
                               int n_blocks, hipStream_t stream);
-hipError_t launch_adj_update_bwd(const KParams &kp, const double *G, const double *Vbar, double *Gbar,
+// Vbar is read and left zeroed (every node).
+hipError_t launch_adj_update_bwd(const KParams &kp, const double *G, double *Vbar, double *Gbar,
                                  const double c[3], const double fbar[3], hipStream_t stream);
 hipError_t launch_adj_p2g_bwd(const KParams &kp, PtsView s0, const double *Gbar, PtsView part, PtsView bar,
                               double *theta, int n, hipStream_t stream);

@@ -131,7 +131,7 @@ __device__ __forceinline__ void node_forward(const KParams &kp, const double c[3
 // Reverse of the grid update over every node of the stored grid: Gbar
 // (m_bar, mv_bar; all nodes written) from Vbar (lines 1-3) and the force
 // adjoint fbar of this substep. G: the grid after P2G (m, mv).
-__global__ __launch_bounds__(kAdjThreads) void k_adj_update_bwd(KParams kp, const double *G, const double *Vbar,
+__global__ __launch_bounds__(kAdjThreads) void k_adj_update_bwd(KParams kp, const double *G, double *Vbar,
                                                                 double *Gbar, double cx, double cy, double cz,
                                                                 double fb0, double fb1, double fb2) {
     const long long nn = kp.grid_pitch;
@@ -140,7 +140,10 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_update_bwd(KParams kp, cons
     const double mass = G[idx];
     if (mass == 0.0) {
 #pragma unroll
-        for (int l = 0; l < 4; l++) Gbar[l * nn + idx] = 0.0;
+        for (int l = 0; l < 4; l++) {
+            Gbar[l * nn + idx] = 0.0;
+            Vbar[l * nn + idx] = 0.0;
+        }
         return;
     }
     const long long b = idx >> 6;
@@ -157,6 +160,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_update_bwd(KParams kp, cons
     double u3b[3], u2b[3], u1b[3], mb = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; i++) u3b[i] = nf.zero[i] ? 0.0 : Vbar[(1 + i) * nn + idx];
+    // this kernel is v_bar's only reader: leave it zeroed for the next substep
+#pragma unroll
+    for (int l = 0; l < 4; l++) Vbar[l * nn + idx] = 0.0;
     if (nf.contact) {
         // force = (u2 - u3) m / dt, u3 = u2 - n n.(u2 - v_ind)
 #pragma unroll
@@ -670,7 +676,7 @@ hipError_t launch_adj_g2p_bwd(const KParams &kp, PtsView s0, PtsView s1, PtsView
     return hipGetLastError();
 }
 
-hipError_t launch_adj_update_bwd(const KParams &kp, const double *G, const double *Vbar, double *Gbar,
+hipError_t launch_adj_update_bwd(const KParams &kp, const double *G, double *Vbar, double *Gbar,
                                  const double c[3], const double fbar[3], hipStream_t stream) {
     k_adj_update_bwd<<<blocks_for(kp.grid_pitch), kAdjThreads, 0, stream>>>(kp, G, Vbar, Gbar, c[0], c[1], c[2],
                                                                             fbar[0], fbar[1], fbar[2]);
