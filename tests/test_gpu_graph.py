@@ -10,7 +10,9 @@ multi-substep parity tests (the larger of 1e-12 and 10x the oracle's own
 rounding spread), utility bits and the simulation time exactly. The record
 period (UpdateEveryNthStep = 4) does not divide the 6-substep calls, so the
 substeps that record p/q move within the captured runs from call to call. A
-change of the elastic constants between calls must capture anew.
+change of the elastic constants between calls must capture anew. Fused runs
+between re-sorts every 3 substeps, unfused single substeps and fused runs of
+a whole call (no re-sort) are covered.
 """
 from __future__ import annotations
 
@@ -55,7 +57,8 @@ def _check(back, ref, spread, n, tag):
     assert np.array_equal(back.utility()[:n], ref.utility()[:n]), tag
 
 
-def test_graph_replays_match_launched_path_and_oracle(require_gpu):
+@pytest.mark.parametrize("fuse,period", [(True, 3), (False, 3), (True, 0)])
+def test_graph_replays_match_launched_path_and_oracle(require_gpu, fuse, period):
     p = scenes.make_params(MID)
     p.UpdateEveryNthStep = 4
     shape, center, vel = MID.indenter()
@@ -65,7 +68,8 @@ def test_graph_replays_match_launched_path_and_oracle(require_gpu):
     gr = _gpu_with(p, hs.copy(), indenter=(shape, center, vel))
     ea = _gpu_with(p, hs.copy(), indenter=(shape, center, vel))
     for g in (gr, ea):
-        g.set_sort_period(3)
+        g.set_sort_period(period)
+        g.set_fusion(fuse)
         g.reset_indenter_force_accumulator()
     gr.set_graph(True)
     gr.enable_timing(True)
@@ -106,9 +110,13 @@ def test_graph_replays_match_launched_path_and_oracle(require_gpu):
     assert normwise(fd, fr) <= 1e-9 and normwise(fe, fr) <= 1e-9, (fd, fe, fr)
     captures, kept = gr.graph_captures()
     replays = gr.read_timings()["graph"][1]
-    # 30 substeps in runs of 3: 10 replays from a handful of captures
-    assert replays == 10, replays
-    assert 1 <= captures <= 6 and kept <= 16, (captures, kept)
+    # 30 substeps in runs of 3 (fused, re-sort every 3), of 1 (unfused) or of
+    # a whole 6-substep call (no re-sort), replayed from a handful of captures
+    run = 3 if (fuse and period) else (1 if not fuse else 6)
+    assert replays == 30 // run, (replays, run)
+    # distinct starting states: at most the runs of one call, once before
+    # and once after the parameter change, plus the first call's
+    assert 1 <= captures <= 2 * (6 // run) + 2 and kept <= 16, (captures, kept)
     print("graph captures", captures, "kept", kept, "replays", replays)
     gr.set_graph(False)
     assert gr.graph_captures()[1] == 0
