@@ -277,9 +277,132 @@ void compute_pq(Point &p, double kappa, double mu) {
 }
 
 // ComputeSVD (mpm_kernels.cuh:486-492)
+// ---------------------------------------------------------------- FP32 SVD
+// The reference computes its 3x3 SVD in single precision with the method of
+// McAdams, Selle, Tamstorf, Teran and Sifakis, "Computing the Singular Value
+// Decomposition of 3x3 matrices with minimal branching and elementary
+// floating point operations" (UW-Madison CS TR1690, 2011): Jacobi
+// eigen-analysis of A^T A with approximate Givens rotations (a fixed number of
+// sweeps), columns of A V sorted by decreasing norm with V kept a rotation,
+// then a QR factorisation by Givens rotations giving U and the singular
+// values (the last one negative for det A < 0). This is that method, written
+// from the paper in float arithmetic, with the rotations accumulated as
+// matrices; it is not bit-faithful to the reference (parity unpinned) and
+// serves to measure how far an FP32 SVD moves the step from the FP64 one.
+bool g_svd_fp32 = false;
+
+// Approximate Givens rotation (cos, sin) that nearly diagonalises the
+// symmetric 2x2 block [[app, apq], [apq, aqq]] (the paper's cheap angle, or
+// pi/8 when that angle would be too coarse).
+void approx_givens(float app, float apq, float aqq, float &c, float &s) {
+    const float gamma = 5.82842712474619f; // 3 + 2 sqrt(2)
+    float ch = 2.0f * (app - aqq), sh = apq;
+    if (gamma * sh * sh < ch * ch) {
+        const float w = 1.0f / std::sqrt(ch * ch + sh * sh);
+        ch *= w;
+        sh *= w;
+    } else {
+        ch = 0.92387953251128674f; // cos(pi/8)
+        sh = 0.38268343236508977f; // sin(pi/8)
+    }
+    c = ch * ch - sh * sh;
+    s = 2.0f * sh * ch;
+}
+
+// QR Givens rotation zeroing a2 against a1 (the paper's robust variant).
+void qr_givens(float a1, float a2, float &c, float &s) {
+    const float eps = 1e-6f;
+    const float rho = std::sqrt(a1 * a1 + a2 * a2);
+    float sh = rho > eps ? a2 : 0.0f;
+    float ch = std::fabs(a1) + std::fmax(rho, eps);
+    if (a1 < 0.0f) std::swap(sh, ch);
+    const float w = 1.0f / std::sqrt(ch * ch + sh * sh);
+    ch *= w;
+    sh *= w;
+    c = ch * ch - sh * sh;
+    s = 2.0f * ch * sh;
+}
+
+void svd3_fp32(const Mat3 &F, Mat3 &U, double S[3], Mat3 &V) {
+    float A[3][3], M[3][3], Vf[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            A[r][c] = (float)F(r, c);
+            Vf[r][c] = r == c ? 1.0f : 0.0f;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
+    // Jacobi sweeps: M <- G^T M G in the (p, q) plane, V <- V G
+    const int pairs[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+    for (int sweep = 0; sweep < 4; sweep++)
+        for (auto &pq : pairs) {
+            const int p = pq[0], q = pq[1];
+            float c, s;
+            approx_givens(M[p][p], M[p][q], M[q][q], c, s);
+            for (int k = 0; k < 3; k++) { // columns
+                const float mp = M[k][p], mq = M[k][q];
+                M[k][p] = c * mp + s * mq;
+                M[k][q] = -s * mp + c * mq;
+            }
+            for (int k = 0; k < 3; k++) { // rows
+                const float mp = M[p][k], mq = M[q][k];
+                M[p][k] = c * mp + s * mq;
+                M[q][k] = -s * mp + c * mq;
+            }
+            for (int k = 0; k < 3; k++) {
+                const float vp = Vf[k][p], vq = Vf[k][q];
+                Vf[k][p] = c * vp + s * vq;
+                Vf[k][q] = -s * vp + c * vq;
+            }
+        }
+    // B = A V, columns sorted by decreasing norm (a swap negates one column
+    // so that V stays a rotation)
+    float B[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) B[r][c] = A[r][0] * Vf[0][c] + A[r][1] * Vf[1][c] + A[r][2] * Vf[2][c];
+    auto norm2 = [&](int c) { return B[0][c] * B[0][c] + B[1][c] * B[1][c] + B[2][c] * B[2][c]; };
+    auto swap_cols = [&](int i, int j) {
+        for (int r = 0; r < 3; r++) {
+            const float b = B[r][i], v = Vf[r][i];
+            B[r][i] = -B[r][j];
+            Vf[r][i] = -Vf[r][j];
+            B[r][j] = b;
+            Vf[r][j] = v;
+        }
+    };
+    if (norm2(0) < norm2(1)) swap_cols(0, 1);
+    if (norm2(0) < norm2(2)) swap_cols(0, 2);
+    if (norm2(1) < norm2(2)) swap_cols(1, 2);
+    // QR: Givens rotations on rows (1,0), (2,0), (2,1); U accumulates their transposes
+    float Uf[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+    const int rows[3][3] = {{0, 1, 0}, {0, 2, 0}, {1, 2, 1}}; // (pivot row, zeroed row, column)
+    for (auto &rr : rows) {
+        const int i = rr[0], j = rr[1], col = rr[2];
+        float c, s;
+        qr_givens(B[i][col], B[j][col], c, s);
+        for (int k = 0; k < 3; k++) {
+            const float bi = B[i][k], bj = B[j][k];
+            B[i][k] = c * bi + s * bj;
+            B[j][k] = -s * bi + c * bj;
+        }
+        for (int k = 0; k < 3; k++) {
+            const float ui = Uf[k][i], uj = Uf[k][j];
+            Uf[k][i] = c * ui + s * uj;
+            Uf[k][j] = -s * ui + c * uj;
+        }
+    }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            U(r, c) = Uf[r][c];
+            V(r, c) = Vf[r][c];
+        }
+    for (int k = 0; k < 3; k++) S[k] = B[k][k];
+}
+
 void compute_svd(Point &p, double kappa, double mu) {
     (void)kappa;
-    svd3(p.Fe, p.U, p.vSigma, p.V);
+    if (g_svd_fp32) svd3_fp32(p.Fe, p.U, p.vSigma, p.V);
+    else svd3(p.Fe, p.U, p.vSigma, p.V);
     for (int i = 0; i < 3; i++) p.vSigmaSquared[i] = p.vSigma[i] * p.vSigma[i];
     const double Je_tr_sq = p.Je_tr * p.Je_tr;
     double s = mu * rcbrt(Je_tr_sq);
@@ -464,6 +587,16 @@ int p2g_particle(const icempm_SimParams3D &P, const double *soa, int pitch, int 
 } // namespace
 
 extern "C" {
+
+void oracle_set_svd_fp32(int on) { g_svd_fp32 = on != 0; }
+
+void oracle_svd3_fp32(const double F[9], double U[9], double S[3], double V[9]) {
+    Mat3 f, u, v;
+    std::memcpy(f.a, F, sizeof(f.a));
+    svd3_fp32(f, u, S, v);
+    std::memcpy(U, u.a, sizeof(u.a));
+    std::memcpy(V, v.a, sizeof(v.a));
+}
 
 void oracle_set_damage(const icempm_Damage *damage) {
     g_damage = damage ? *damage : icempm_Damage{0, 1.0, 1.0};

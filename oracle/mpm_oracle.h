@@ -44,6 +44,13 @@ void oracle_stable_sort(const uint32_t *keys, int n, int32_t *perm);
  * compacts enabled points (stable), then sorts by reference key. Returns new size. */
 int oracle_remove_disabled_and_sort(const icempm_SimParams3D *prms, double *soa, int pitch, int n);
 
+/* SVD in single precision by the method the reference uses (McAdams et al.
+ * 2011), written from the paper; not bit-faithful (parity unpinned). With
+ * oracle_set_svd_fp32(1) the return mapping uses it instead of the FP64 SVD
+ * (process-wide), to measure the gap between the two. */
+void oracle_set_svd_fp32(int on);
+void oracle_svd3_fp32(const double F[9], double U[9], double S[3], double V[9]);
+
 /* The damage model of the return mapping (no reference counterpart; see
  * icempm_Damage). Process-wide; NULL disables it. */
 void oracle_set_damage(const icempm_Damage *damage);

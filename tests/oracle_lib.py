@@ -41,6 +41,8 @@ _PROTOS = {
     "oracle_adj_forward": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "oracle_adj_gradient": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "oracle_set_damage": (None, [C.POINTER(capi.Damage)]),
+    "oracle_set_svd_fp32": (None, [C.c_int]),
+    "oracle_svd3_fp32": (None, [_dp, _dp, _dp, _dp]),
 }
 
 ORACLE_LIB_NOFMA = ROOT / "oracle" / "lib" / "libmpm_oracle_nofma.so"
