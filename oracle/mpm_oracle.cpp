@@ -1,7 +1,7 @@
 // mpm_oracle.cpp — CPU restatement of the reference MPM step (see mpm_oracle.h).
 //
-// TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the host-core
-// baseline. Each function names the reference lines it restates. Arithmetic
+// TEST INFRASTRUCTURE ONLY: the reference against which the HIP path is
+// tested, and the host-core baseline. Each function names the reference lines it restates. Arithmetic
 // follows the reference's expression order so that the only differences to
 // the device are rounding-level (FMA contraction, libm cbrt/sqrt ulps,
 // atomic summation order).

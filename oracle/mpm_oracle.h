@@ -1,9 +1,9 @@
 /*
  * mpm_oracle.h — CPU restatement of the reference's single-partition MPM step.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP path: it
- * is loaded by tests/, by __graft_entry__.smoke() and by bench.py's
- * cpu_baseline leg, and nothing in the product links or calls it.
+ * TEST INFRASTRUCTURE ONLY. This library checks the HIP path: it is loaded by
+ * tests/, by the repository's smoke test and by bench.py's host-core
+ * baseline, and nothing in the product links or calls it.
  *
  * Every function restates a reference function (file:line given beside it)
  * in plain C++ on FP64 host arithmetic. Buffers use the reference layouts:

@@ -24,4 +24,4 @@ def gpu_available():
 @pytest.fixture
 def require_gpu(gpu_available):
     if not gpu_available:
-        pytest.fail("no HIP device visible: GPU tests must run on the GPU box")
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/libmpm_oracle.so).
 
-Test infrastructure: the oracle is the checker for the HIP path and is
-never part of the product. Builds the library with oracle/Makefile on first
+Test infrastructure: the oracle is what the HIP path is tested against and
+is never part of the product. Builds the library with oracle/Makefile on first
 use if it is missing.
 """
 from __future__ import annotations
