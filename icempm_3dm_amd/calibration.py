@@ -7,7 +7,10 @@ fixed initial state that returns the indenter force averaged over every
 ``sample_every`` substeps, and the reverse mode of that run, which returns
 dL/d(E, nu, friction angle, cohesion) for a loss L of the force samples.
 ``IndenterForceCurve`` wraps the pair as a ``torch.autograd.Function``;
-``calibrate`` fits parameters to a target curve with Adam.
+``IndenterResponse`` adds the point positions at the end of the run (for a
+displacement misfit; ``icempm_adjoint_positions`` /
+``icempm_adjoint_backward_ex``); ``calibrate`` fits parameters to a target
+curve with Adam.
 
 The differentiated model is the step with its plastic return mapping; the
 crush test and the return mapping's branch choices (tension cap or shear
@@ -101,13 +104,29 @@ class ForceCurveModel:
         self.evaluations += 1
         return out
 
-    def backward(self, dL_dF: np.ndarray, raw: bool = False):
-        """dL/d(E, nu, phi_deg, cohesion) of the last forward for dL/dF (n_samples, 3);
-        with raw, also dL/d(mu, kappa, DP_threshold_p, DP_tan_phi)."""
-        w = np.ascontiguousarray(dL_dF, dtype=np.float64).reshape(self.n_samples, 3)
+    def positions(self) -> np.ndarray:
+        """Point positions (3, n) at the end of the last forward, in the order of
+        the uploaded points."""
+        n = self.gpu.hssoa.size
+        out = np.zeros((3, n))
+        check(lib().icempm_adjoint_positions(self.gpu.ctx, dptr(out)))
+        return out
+
+    def backward(self, dL_dF: np.ndarray | None, dL_dx: np.ndarray | None = None, raw: bool = False):
+        """dL/d(E, nu, phi_deg, cohesion) of the last forward for dL/dF
+        (n_samples, 3) and dL/dx (3, n) of the final positions (either may be
+        None); with raw, also dL/d(mu, kappa, DP_threshold_p, DP_tan_phi)."""
         g = np.zeros(4)
         r = np.zeros(4)
-        check(lib().icempm_adjoint_backward(self.gpu.ctx, dptr(w), dptr(g), dptr(r)))
+        w = None if dL_dF is None else np.ascontiguousarray(dL_dF, dtype=np.float64).reshape(self.n_samples, 3)
+        if dL_dx is None:
+            if w is None:
+                w = np.zeros((self.n_samples, 3))
+            check(lib().icempm_adjoint_backward(self.gpu.ctx, dptr(w), dptr(g), dptr(r)))
+        else:
+            x = np.ascontiguousarray(dL_dx, dtype=np.float64).reshape(3, self.gpu.hssoa.size)
+            check(lib().icempm_adjoint_backward_ex(self.gpu.ctx, None if w is None else dptr(w), dptr(x), dptr(g),
+                                                   dptr(r)))
         return (g, r) if raw else g
 
     def close(self) -> None:
@@ -140,6 +159,35 @@ class IndenterForceCurve:
                 def backward(ctx, gF):
                     g = ctx.model.backward(gF.detach().cpu().double().numpy())
                     return torch.from_numpy(g).to(gF.dtype), None
+
+            cls._fn = _Fn
+        return cls._fn.apply(theta, model)
+
+
+class IndenterResponse:
+    """torch.autograd.Function: theta (4,) float64 -> (force samples (n, 3),
+    final point positions (3, n_points)), for force and displacement misfits."""
+
+    _fn = None
+
+    @classmethod
+    def apply(cls, theta, model: ForceCurveModel):
+        if cls._fn is None:
+            torch = _torch()
+
+            class _Fn(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, th, mdl):
+                    ctx.model = mdl
+                    f = mdl.forward(th.detach().cpu().numpy())
+                    x = mdl.positions()
+                    return torch.from_numpy(f).to(th.dtype), torch.from_numpy(x).to(th.dtype)
+
+                @staticmethod
+                def backward(ctx, gF, gX):
+                    g = ctx.model.backward(None if gF is None else gF.detach().cpu().double().numpy(),
+                                           None if gX is None else gX.detach().cpu().double().numpy())
+                    return torch.from_numpy(g).to(gF.dtype if gF is not None else gX.dtype), None
 
             cls._fn = _Fn
         return cls._fn.apply(theta, model)

@@ -231,6 +231,9 @@ _PROTOS = {
     "icempm_set_blocks_per_cu": (C.c_int, [_ctxp, C.c_int]),
     "icempm_adjoint_forward": (C.c_int, [_ctxp, C.POINTER(AdjointRun), _dp]),
     "icempm_adjoint_backward": (C.c_int, [_ctxp, _dp, _dp, _dp]),
+    "icempm_adjoint_positions": (C.c_int, [_ctxp, C.POINTER(C.c_double)]),
+    "icempm_adjoint_backward_ex": (C.c_int, [_ctxp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "icempm_scene_block": (C.c_int, [_dp, C.c_uint, _ip, C.c_int, _dp, C.c_double, C.c_double,
                                      C.c_ulonglong, C.c_int, _ip]),
     "icempm_read_raw_points": (C.c_int, [C.c_char_p, _dp, C.c_uint, _ip, _dp]),

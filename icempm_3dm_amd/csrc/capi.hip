@@ -152,6 +152,10 @@ struct icempm_ctx {
         double *forces = nullptr;
         double *theta = nullptr; // adjoints of mu, kappa, DP_threshold_p, DP_tan_phi (after the force samples)
         int *flags = nullptr;    // crushed / liquid points seen
+        double *xfin = nullptr;  // final positions of the forward run (3 lines of n, device order)
+        double *dx = nullptr;    // 3 lines of n: positions or their adjoints in upload order
+        int32_t *slot = nullptr; // upload index of each device point at the forward run
+        size_t x_cap = 0, dx_cap = 0, slot_cap = 0;
         std::vector<double> centers, times; // indenter center and time per substep
         size_t ckpt_cap = 0, traj_cap = 0, grid_cap = 0, pt_cap = 0, f_cap = 0;
         bool have_forward = false;
@@ -396,8 +400,9 @@ void free_all(icempm_ctx *ctx) {
     {
         auto &A = ctx->adj;
         // theta lives inside the forces allocation
-        for (double *q : {A.ckpt, A.traj, A.G, A.V, A.Gbar, A.vbar, A.bar, A.part, A.forces})
+        for (double *q : {A.ckpt, A.traj, A.G, A.V, A.Gbar, A.vbar, A.bar, A.part, A.forces, A.xfin, A.dx})
             if (q) (void)hipFree(q);
+        if (A.slot) (void)hipFree(A.slot);
         for (int32_t *q : {A.vbar_flag, A.vbar_list, A.vbar_count})
             if (q) (void)hipFree(q);
         if (A.flags) (void)hipFree(A.flags);
@@ -1608,17 +1613,53 @@ int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double
         if (K & 1) std::swap(cur, nxt);
     }
     ICEMPM_HIP(launch_adj_flags(adj_view(cur, n), n, A.flags, ctx->stream));
+    // the final positions and the point order, for position losses
+    ICEMPM_TRY(adj_reserve(&A.xfin, A.x_cap, 3 * (size_t)std::max(n, 1), "final positions"));
+    ICEMPM_TRY(adj_reserve(&A.dx, A.dx_cap, 3 * (size_t)std::max(n, 1), "position adjoints"));
+    if (A.slot_cap < (size_t)std::max(n, 1)) {
+        if (A.slot) (void)hipFree(A.slot);
+        A.slot = nullptr;
+        A.slot_cap = 0;
+        ICEMPM_TRY(dev_alloc(&A.slot, (size_t)std::max(n, 1), "point order"));
+        A.slot_cap = (size_t)std::max(n, 1);
+    }
+    if (n > 0) {
+        ICEMPM_HIP(hipMemcpyAsync(A.xfin, cur + (size_t)ICEMPM_POSX * n, sizeof(double) * 3 * n,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+        ICEMPM_HIP(hipMemcpyAsync(A.slot, ctx->slot[ctx->cur_pts], sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+    }
     ICEMPM_HIP(hipMemcpyAsync(forces_out, A.forces, sizeof(double) * 3 * nsamp, hipMemcpyDeviceToHost, ctx->stream));
     ICEMPM_TRY(adj_check(ctx));
     A.have_forward = true;
     return ICEMPM_OK;
 }
 
-int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[4]) {
+int icempm_adjoint_positions(icempm_ctx *ctx, double *positions_out) {
     ICEMPM_TRY(check_ready(ctx));
     auto &A = ctx->adj;
     if (!A.have_forward) return fail(ICEMPM_ERR_NOT_READY, "icempm_adjoint_forward has not completed");
-    if (!dL_dforces || !grad_out) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    if (!positions_out) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    const int n = A.n;
+    if (n > 0) {
+        ICEMPM_HIP(launch_lines_by_slot(A.xfin, A.slot, A.dx, n, 3, true, ctx->stream));
+        ICEMPM_HIP(hipMemcpyAsync(positions_out, A.dx, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
+    return ICEMPM_OK;
+}
+
+int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[4]) {
+    if (!dL_dforces) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
+    return icempm_adjoint_backward_ex(ctx, dL_dforces, nullptr, grad_out, raw_out);
+}
+
+int icempm_adjoint_backward_ex(icempm_ctx *ctx, const double *dL_dforces, const double *dL_dpositions,
+                               double grad_out[4], double raw_out[4]) {
+    ICEMPM_TRY(check_ready(ctx));
+    auto &A = ctx->adj;
+    if (!A.have_forward) return fail(ICEMPM_ERR_NOT_READY, "icempm_adjoint_forward has not completed");
+    if (!grad_out) return fail(ICEMPM_ERR_INVALID_ARG, "null argument");
     if (A.n != ctx->n) return fail(ICEMPM_ERR_NOT_READY, "the points changed since icempm_adjoint_forward");
     const int K = A.run.checkpoint_every, n = A.n, S = A.run.sample_every;
     const long long gp = ctx->kp.grid_pitch;
@@ -1626,6 +1667,11 @@ int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double gr
     ICEMPM_HIP(hipMemsetAsync(A.bar, 0, sizeof(double) * st, ctx->stream));
     ICEMPM_HIP(hipMemsetAsync(A.part, 0, sizeof(double) * st, ctx->stream));
     ICEMPM_HIP(hipMemsetAsync(A.theta, 0, sizeof(double) * 4, ctx->stream));
+    if (dL_dpositions && n > 0) {
+        // the adjoint of the final state: dL/dx_N, from upload order into the device order
+        ICEMPM_HIP(hipMemcpyAsync(A.dx, dL_dpositions, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+        ICEMPM_HIP(launch_lines_by_slot(A.dx, A.slot, A.bar + (size_t)ICEMPM_POSX * n, n, 3, false, ctx->stream));
+    }
     const PtsView bar = adj_view(A.bar, n), part = adj_view(A.part, n);
     SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
     GridState gb;
@@ -1647,7 +1693,7 @@ int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double gr
         for (int j = K - 1; j >= 0; j--) {
             const int s = s0 + j;
             double fbar[3];
-            for (int i = 0; i < 3; i++) fbar[i] = dL_dforces[3 * (s / S) + i] / S;
+            for (int i = 0; i < 3; i++) fbar[i] = dL_dforces ? dL_dforces[3 * (s / S) + i] / S : 0.0;
             const PtsView s_0 = adj_view(A.traj + st * j, n), s_1 = adj_view(A.traj + st * (j + 1), n);
             const double *G = A.G + (size_t)4 * gp * j, *V = A.V + (size_t)4 * gp * j;
             ICEMPM_HIP(launch_adj_g2p_bwd(ctx->kp, s_0, s_1, bar, part, n, seg, V, gb, 0, A.theta, A.any_crushed,

@@ -126,6 +126,11 @@ hipError_t launch_adj_p2g_bwd(const KParams &kp, PtsView s0, const double *Gbar,
                               double *theta, int n, hipStream_t stream);
 // out[0..2] += scale * (sum of the indenter accumulators per component).
 hipError_t launch_sum_bins(const double *bins, long long nbins, double scale, double *out, hipStream_t stream);
+// `lines` point lines of n doubles between the device order and the upload
+// order (slot: the upload index of each device point, a permutation of
+// 0..n-1): to_upload: out[slot[i]] = in[i]; otherwise out[i] = in[slot[i]].
+hipError_t launch_lines_by_slot(const double *in, const int32_t *slot, double *out, int n, int lines, bool to_upload,
+                                hipStream_t stream);
 // flags[0] |= 1 if a point is crushed, flags[1] |= 1 if a point is liquid.
 hipError_t launch_adj_flags(PtsView st, int n, int *flags, hipStream_t stream);
 

@@ -662,6 +662,20 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_flags(PtsView st, int n, in
 
 inline unsigned blocks_for(long long n) { return (unsigned)((n + kAdjThreads - 1) / kAdjThreads); }
 
+// Point lines between the device order and the upload order: with
+// to_upload, out line l [slot[i]] = in line l [i]; otherwise out line l [i] =
+// in line l [slot[i]] (lines of n doubles; slot is a permutation of 0..n-1).
+__global__ void k_lines_by_slot(const double *in, const int32_t *slot, double *out, int n, int lines,
+                                int to_upload) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = slot[i];
+    for (int l = 0; l < lines; l++) {
+        if (to_upload) out[(long long)l * n + k] = in[(long long)l * n + i];
+        else out[(long long)l * n + i] = in[(long long)l * n + k];
+    }
+}
+
 } // namespace
 
 hipError_t launch_adj_g2p_bwd(const KParams &kp, PtsView s0, PtsView s1, PtsView bar, PtsView part, int n,
@@ -698,6 +712,13 @@ hipError_t launch_sum_bins(const double *bins, long long nbins, double scale, do
 hipError_t launch_adj_flags(PtsView st, int n, int *flags, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     k_adj_flags<<<blocks_for(n), kAdjThreads, 0, stream>>>(st, n, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_lines_by_slot(const double *in, const int32_t *slot, double *out, int n, int lines, bool to_upload,
+                                hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_lines_by_slot<<<(n + 255) / 256, 256, 0, stream>>>(in, slot, out, n, lines, to_upload ? 1 : 0);
     return hipGetLastError();
 }
 

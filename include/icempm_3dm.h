@@ -380,6 +380,15 @@ int icempm_adjoint_forward(icempm_ctx *ctx, const icempm_AdjointRun *run, double
  * last two are exactly 0 when no point is crushed. raw_out[4] (may be NULL):
  * dL/d(mu, kappa, DP_threshold_p, DP_tan_phi). */
 int icempm_adjoint_backward(icempm_ctx *ctx, const double *dL_dforces, double grad_out[4], double raw_out[4]);
+/* The point positions at the end of the last icempm_adjoint_forward: 3 lines
+ * of n doubles (x, y, z), points in upload order (as given to
+ * icempm_transfer_points_to_device). Synchronizes. */
+int icempm_adjoint_positions(icempm_ctx *ctx, double *positions_out);
+/* icempm_adjoint_backward for a loss of the force samples and of those final
+ * positions (a displacement misfit): dL_dpositions holds dL/dx_N in the
+ * layout of icempm_adjoint_positions. Either input may be NULL (zero). */
+int icempm_adjoint_backward_ex(icempm_ctx *ctx, const double *dL_dforces, const double *dL_dpositions,
+                               double grad_out[4], double raw_out[4]);
 
 /* Synthetic input (no reference counterpart; replaces the HDF5 point cloud of
  * SnapshotManager::LoadRawPoints for generated scenes): a block of

@@ -1218,6 +1218,12 @@ int oracle_adj_forward(const icempm_SimParams3D *prms, const icempm_Indenter *in
 
 int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
                         int n, int n_steps, int sample_every, const double *dL_dF, double *grad_mu_kappa) {
+    return oracle_adj_gradient_ex(prms, ind, soa0, pitch, n, n_steps, sample_every, dL_dF, nullptr, grad_mu_kappa);
+}
+
+int oracle_adj_gradient_ex(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
+                           int n, int n_steps, int sample_every, const double *dL_dF, const double *dL_dx,
+                           double *grad_mu_kappa) {
     if (g_damage.enabled) return ICEMPM_ADJ_UNSUPPORTED; // the damage model is not differentiated
     // grad_mu_kappa: dL/d(mu, kappa, DP_threshold_p, DP_tan_phi)
     const icempm_SimParams3D &P = *prms;
@@ -1248,9 +1254,14 @@ int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *i
             if (util_bits(cur[i]) & ICEMPM_STATUS_LIQUID) return kAdjUnsupported;
     }
     // reverse
+    // the adjoint of the final state: dL/dx_N (3 lines of n) or zero
     std::vector<PtState> bar(n);
-    for (auto &b : bar) {
-        for (int a = 0; a < 3; a++) b.x[a] = b.v[a] = 0.0;
+    for (int p = 0; p < n; p++) {
+        PtState &b = bar[p];
+        for (int a = 0; a < 3; a++) {
+            b.x[a] = dL_dx ? dL_dx[(size_t)a * n + p] : 0.0;
+            b.v[a] = 0.0;
+        }
         b.F = zero3();
         b.C = zero3();
     }
@@ -1263,7 +1274,7 @@ int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *i
         const double *center = &centers[3 * s];
         double fbar[3] = {0, 0, 0};
         if (s / sample_every < nsamp)
-            for (int i = 0; i < 3; i++) fbar[i] = dL_dF[3 * (s / sample_every) + i] / sample_every;
+            for (int i = 0; i < 3; i++) fbar[i] = dL_dF ? dL_dF[3 * (s / sample_every) + i] / sample_every : 0.0;
         // recompute the grid of substep s
         std::fill(gm.begin(), gm.end(), 0.0);
         oracle_p2g(prms, st, pitch, n, gm.data(), 1);

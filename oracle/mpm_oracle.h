@@ -94,6 +94,12 @@ int oracle_adj_forward(const icempm_SimParams3D *prms, const icempm_Indenter *in
  * DP_threshold_p, DP_tan_phi). Returns 91 for liquid points. */
 int oracle_adj_gradient(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
                         int n, int n_steps, int sample_every, const double *dL_dF, double *grad_mu_kappa);
+/* The same with a loss that also depends on the final positions: dL_dx holds
+ * dL/dx_N as 3 lines of n doubles (x, y, z; point order of soa0). Either
+ * dL_dF or dL_dx may be NULL (zero). */
+int oracle_adj_gradient_ex(const icempm_SimParams3D *prms, const icempm_Indenter *ind, const double *soa0, int pitch,
+                           int n, int n_steps, int sample_every, const double *dL_dF, const double *dL_dx,
+                           double *grad_mu_kappa);
 
 #ifdef __cplusplus
 }

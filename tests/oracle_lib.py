@@ -40,6 +40,7 @@ _PROTOS = {
     "oracle_step": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, C.c_int]),
     "oracle_adj_forward": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "oracle_adj_gradient": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]),
+    "oracle_adj_gradient_ex": (C.c_int, [_pp, _indp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
     "oracle_set_damage": (None, [C.POINTER(capi.Damage)]),
     "oracle_set_svd_fp32": (None, [C.c_int]),
     "oracle_svd3_fp32": (None, [_dp, _dp, _dp, _dp]),
@@ -188,4 +189,16 @@ def adj_gradient(p: SimParams3D, ind: Indenter, soa: np.ndarray, pitch: int, n: 
     w = np.ascontiguousarray(dL_dF, dtype=np.float64)
     code = lib().oracle_adj_gradient(C.byref(p), C.byref(ind), _d(soa), pitch, n, n_steps, sample_every, _d(w),
                                      _d(g))
+    return code, g[0], g[1], g[2], g[3]
+
+
+def adj_gradient_ex(p: SimParams3D, ind: Indenter, soa: np.ndarray, pitch: int, n: int, n_steps: int,
+                    sample_every: int, dL_dF, dL_dx):
+    """adj_gradient with a final-position term: dL_dx (3, n) = dL/dx_N in point
+    order; either may be None. Returns (code, 4 parameter adjoints)."""
+    g = np.zeros(4)
+    w = None if dL_dF is None else np.ascontiguousarray(dL_dF, dtype=np.float64)
+    x = None if dL_dx is None else np.ascontiguousarray(dL_dx, dtype=np.float64).reshape(3, n)
+    code = lib().oracle_adj_gradient_ex(C.byref(p), C.byref(ind), _d(soa), pitch, n, n_steps, sample_every,
+                                        None if w is None else _d(w), None if x is None else _d(x), _d(g))
     return code, g[0], g[1], g[2], g[3]

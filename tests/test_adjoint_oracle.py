@@ -140,3 +140,45 @@ def test_plastic_gradient_matches_central_differences(name, idx):
         vals.append(_loss(q, ind, hs, w))
     fd = (vals[0] - vals[1]) / (2 * d)
     assert abs(g[idx] - fd) <= 1e-5 * abs(fd), (name, g[idx], fd)
+
+
+def _final_positions(p, ind, hs):
+    r = hs.copy()
+    code, _ = orc.step(p, ind, r.host_buffer, r.capacity, r.size, orc.grid_zeros(p), np.zeros(p.IndenterArraySize()),
+                       N_STEPS, 0.0, 0, 1)
+    assert code == 0
+    return r.lines()[capi.POSX:capi.POSX + 3, :r.size].copy()
+
+
+@pytest.mark.parametrize("with_forces", [False, True])
+def test_displacement_misfit_gradient_matches_central_differences(with_forces):
+    """L = sum wx . (x_N - x_0) [+ sum wF . F]: the final-position adjoint
+    seeds the reverse pass. Central differences at relative step 3e-4: the
+    positions barely depend on kappa over 40 substeps, so smaller steps leave
+    the quotient to the positions' rounding (1.4e-6 at 3e-5, 2.8e-6 at 1e-4),
+    while the truncation error is still 4e-7 at 1e-3."""
+    p, ind, hs = _setup((8e9, 0.3, 55.0, 1.5e6))
+    n = hs.size
+    rng = np.random.default_rng(11)
+    wx = rng.standard_normal((3, n))
+    wF = rng.standard_normal((N_STEPS // S, 3)) if with_forces else None
+    x0 = hs.lines()[capi.POSX:capi.POSX + 3, :n].copy()
+
+    def loss(pp):
+        val = float((wx * (_final_positions(pp, ind, hs) - x0)).sum())
+        if with_forces:
+            val += _loss(pp, ind, hs, wF)
+        return val
+
+    code, gmu, gk, _, _ = orc.adj_gradient_ex(p, ind, hs.host_buffer, hs.capacity, n, N_STEPS, S, wF, wx)
+    assert code == 0
+    for name, g in (("mu", gmu), ("kappa", gk)):
+        v0 = getattr(p, name)
+        d = v0 * 3e-4
+        vals = []
+        for sgn in (1, -1):
+            q, _, _ = _setup((8e9, 0.3, 55.0, 1.5e6))
+            setattr(q, name, v0 + sgn * d)
+            vals.append(loss(q))
+        fd = (vals[0] - vals[1]) / (2 * d)
+        assert abs(g - fd) <= 1e-6 * abs(fd), (name, g, fd)

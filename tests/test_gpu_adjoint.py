@@ -6,7 +6,10 @@ test_adjoint_oracle.py); central finite differences of the device's own
 forward (relative step 1e-6) for dL/dE and dL/dnu; independence of the
 checkpoint period; a plastic run through the return mapping (all four
 parameter adjoints against the oracle and against finite differences); and
-the torch.autograd wrapper and calibration loop.
+the torch.autograd wrapper and calibration loop; and a displacement misfit
+(a loss of the final point positions, alone and with the forces): the final
+positions against the oracle's step, the parameter adjoints against the
+oracle's reverse mode and against central differences of the device forward.
 """
 from __future__ import annotations
 
@@ -17,8 +20,8 @@ import pytest
 
 import oracle_lib as orc
 from icempm_3dm_amd import capi, scenes
-from icempm_3dm_amd.calibration import (ForceCurveModel, IndenterForceCurve, TARGET_THETA, calibrate,
-                                        elastic_params)
+from icempm_3dm_amd.calibration import (ForceCurveModel, IndenterForceCurve, IndenterResponse, TARGET_THETA,
+                                        calibrate, elastic_params)
 from parity_util import normwise
 
 pytestmark = pytest.mark.gpu
@@ -189,4 +192,66 @@ def test_calibration_scene_gradient_and_fit(require_gpu):
     best = min(hist, key=lambda h: h[0])
     print("misfit (E)", [round(h[0], 8) for h in hist], "best theta", best[1])
     assert abs(best[1][0] / TARGET_THETA[0] - 1) < 0.01
+    m.close()
+
+
+@pytest.mark.parametrize("with_forces", [False, True])
+def test_displacement_misfit_gradient(require_gpu, with_forces):
+    n_steps, S, K = 40, 10, 20
+    theta = np.array(TARGET_THETA)
+    m = ForceCurveModel(SMALL, n_steps, S, K, theta=theta)
+    hs = m.gpu.hssoa
+    n = hs.size
+    F = m.forward()
+    X = m.positions()
+    # the final positions are the step's (upload order)
+    p = elastic_params(SMALL, theta)
+    ind = capi.Indenter.make(*SMALL.indenter())
+    ref = hs.copy()
+    code, _ = orc.step(p, ind, ref.host_buffer, ref.capacity, n, orc.grid_zeros(p), np.zeros(p.IndenterArraySize()),
+                       n_steps, 0.0, 0, 1)
+    assert code == 0
+    assert normwise(X, ref.lines()[capi.POSX:capi.POSX + 3, :n]) <= 1e-12
+    rng = np.random.default_rng(12)
+    wx = rng.standard_normal((3, n))
+    wF = rng.standard_normal((n_steps // S, 3)) if with_forces else None
+    g, raw = m.backward(wF, wx, raw=True)
+    code, *g_ref = orc.adj_gradient_ex(p, ind, hs.host_buffer, hs.capacity, n, n_steps, S, wF, wx)
+    assert code == 0
+    assert normwise(raw[:2], np.array(g_ref[:2])) <= 1e-8, (raw, g_ref)
+    # central differences of the device forward (relative step 3e-4: the
+    # positions' own rounding limits smaller steps)
+    x0 = hs.lines()[capi.POSX:capi.POSX + 3, :n].copy()
+
+    def loss(th):
+        f = m.forward(th)
+        val = float((wx * (m.positions() - x0)).sum())
+        return val + (float((wF * f).sum()) if with_forces else 0.0)
+
+    for j in range(2):
+        d = theta[j] * 3e-4
+        vals = []
+        for sgn in (1, -1):
+            th = theta.copy()
+            th[j] += sgn * d
+            vals.append(loss(th))
+        fd = (vals[0] - vals[1]) / (2 * d)
+        assert abs(g[j] - fd) <= 1e-6 * abs(fd), (j, g[j], fd)
+    m.close()
+
+
+def test_torch_response_function(require_gpu):
+    torch = pytest.importorskip("torch")
+    n_steps, S, K = 40, 10, 20
+    m = ForceCurveModel(TINY, n_steps, S, K, theta=TARGET_THETA)
+    n = m.gpu.hssoa.size
+    th = torch.tensor(np.array(TARGET_THETA), dtype=torch.float64, requires_grad=True)
+    F, X = IndenterResponse.apply(th, m)
+    rng = np.random.default_rng(13)
+    wF = torch.from_numpy(rng.standard_normal((n_steps // S, 3)))
+    wx = torch.from_numpy(rng.standard_normal((3, n)))
+    loss = (wF * F).sum() + (wx * X).sum()
+    loss.backward()
+    g_direct = m.backward(wF.numpy(), wx.numpy())
+    assert np.allclose(th.grad.numpy(), g_direct, rtol=1e-12, atol=0)
     m.close()
