@@ -96,6 +96,8 @@ struct icempm_ctx {
     int32_t *seg_start = nullptr;
     int32_t *seg_brick = nullptr;
     int32_t *nseg = nullptr;
+    int32_t *wg_start = nullptr; // segment ranges of the segment kernels' workgroups
+    int split_blocks = 0;        // the grid wg_start was computed for (0: stale)
     int32_t *work = nullptr; // G2P worklists: 2 counts, crushed list, spill list
     int p2g_blocks = 2048;
     int g2p_blocks = 4096;
@@ -412,6 +414,9 @@ void free_all(icempm_ctx *ctx) {
     f(ctx->seg_start);
     f(ctx->seg_brick);
     f(ctx->nseg);
+    f(ctx->wg_start);
+    ctx->wg_start = nullptr;
+    ctx->split_blocks = 0;
     f(ctx->work);
     f(ctx->bins);
     f(ctx->err);
@@ -430,6 +435,16 @@ int check_ready(icempm_ctx *ctx) {
     return set_device(ctx);
 }
 
+// The workgroup split of the segment kernels for the current segments and
+// workgroup count (computed on the device after every change of either).
+int ensure_split(icempm_ctx *ctx) {
+    const int nb = segment_grid(ctx->p2g_blocks);
+    if (ctx->split_blocks == nb) return ICEMPM_OK;
+    ICEMPM_HIP(launch_work_split(ctx->seg_start, ctx->nseg, ctx->wg_start, nb, ctx->stream));
+    ctx->split_blocks = nb;
+    return ICEMPM_OK;
+}
+
 // Regroups device points by brick key and rebuilds the brick segments.
 int brick_sort(icempm_ctx *ctx) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_SORT);
@@ -445,6 +460,8 @@ int brick_sort(icempm_ctx *ctx) {
     }
     ICEMPM_HIP(launch_brick_segments(ctx->keys[0], n, ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->scratch,
                                      ctx->stream));
+    ctx->split_blocks = 0;
+    ICEMPM_TRY(ensure_split(ctx));
     ctx->substeps_since_sort = 0;
     ctx->n_sorts++;
     return tm.finish();
@@ -460,7 +477,8 @@ int do_reset_grid(icempm_ctx *ctx, int gi) {
 
 int do_p2g(icempm_ctx *ctx, int gi) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_P2G);
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    ICEMPM_TRY(ensure_split(ctx));
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
     ICEMPM_HIP(launch_p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->gstate(gi), ctx->grids[gi].cur, ctx->err,
                           ctx->p2g_blocks, ctx->stream));
     return tm.finish();
@@ -498,7 +516,8 @@ int do_update(icempm_ctx *ctx, int gi, double cx, double cy, double cz, double t
 // before cleared the spill count. clocked: the record flag is the clock's.
 int do_g2p(icempm_ctx *ctx, int gi, int record_pq, bool count_zeroed = false, bool clocked = false) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_G2P);
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    ICEMPM_TRY(ensure_split(ctx));
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
     ICEMPM_HIP(launch_g2p(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[gi].grid, record_pq, ctx->err,
                           ctx->work, ctx->g2p_blocks, clocked ? ctx->clock + ctx->clock_slot : nullptr, count_zeroed,
                           ctx->stream));
@@ -512,7 +531,8 @@ int do_g2p(icempm_ctx *ctx, int gi, int record_pq, bool count_zeroed = false, bo
 // G2P, which recompute velocity and C: those lines are not stored.
 int do_g2p2g(icempm_ctx *ctx, int ga, int gb, int record_pq, bool clocked) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_G2P2G);
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    ICEMPM_TRY(ensure_split(ctx));
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
     ICEMPM_HIP(launch_g2p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[ga].grid, ctx->gstate(gb),
                             ctx->grids[gb].cur, record_pq, 0, ctx->err, ctx->work, ctx->p2g_blocks,
                             clocked ? ctx->clock + ctx->clock_slot : nullptr, true, ctx->stream));
@@ -740,6 +760,9 @@ int alloc_common(icempm_ctx *ctx, const icempm_SimParams3D &prms, int n_points_c
     ICEMPM_TRY(dev_alloc(&ctx->seg_start, cap + 1, "segments"));
     ICEMPM_TRY(dev_alloc(&ctx->seg_brick, cap, "segment bricks"));
     ICEMPM_TRY(dev_alloc(&ctx->nseg, 1, "segment count"));
+    // at most 64 workgroups per CU (icempm_set_blocks_per_cu), rounded to the XCDs, plus one
+    ICEMPM_TRY(dev_alloc(&ctx->wg_start, (size_t)segment_grid(64 * ctx->n_cu) + 1, "workgroup split"));
+    ctx->split_blocks = 0;
     ICEMPM_TRY(dev_alloc(&ctx->work, 2 * cap + 2, "G2P worklists"));
     ctx->nbins = nbins;
     ICEMPM_TRY(dev_alloc(&ctx->bins, (size_t)ctx->nbins, "indenter bins"));
@@ -1467,7 +1490,8 @@ int adj_run(icempm_ctx *ctx, int s0, int count, Slot slot, bool keep, bool recor
     auto &A = ctx->adj;
     const long long gp = ctx->kp.grid_pitch;
     const int S = A.run.sample_every;
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    ICEMPM_TRY(ensure_split(ctx));
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
     int ga = 0, gb = 1;
     ICEMPM_TRY(do_reset_grid(ctx, ga));
     ICEMPM_HIP(launch_p2g(ctx->kp, adj_view(slot(0), A.n), A.n, seg, ctx->gstate(ga), ctx->grids[ga].cur, ctx->err,
@@ -1673,7 +1697,8 @@ int icempm_adjoint_backward_ex(icempm_ctx *ctx, const double *dL_dforces, const 
         ICEMPM_HIP(launch_lines_by_slot(A.dx, A.slot, A.bar + (size_t)ICEMPM_POSX * n, n, 3, false, ctx->stream));
     }
     const PtsView bar = adj_view(A.bar, n), part = adj_view(A.part, n);
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg};
+    ICEMPM_TRY(ensure_split(ctx));
+    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
     GridState gb;
     gb.grid = A.vbar;
     gb.brick_flag = A.vbar_flag;

@@ -90,7 +90,16 @@ struct SegInfo {
     const int32_t *seg_start; // nseg+1
     const int32_t *seg_brick;
     const int32_t *nseg;      // device int
+    const int32_t *wg_start;  // first segment of each workgroup of the segment kernels (+1 entry), launch_work_split
 };
+// Workgroups of a segment-kernel launch for a requested count (a multiple of
+// the 8 XCDs).
+inline int segment_grid(int n_blocks) { return (n_blocks + kXcd - 1) / kXcd * kXcd; }
+// wg_start[b] for b in [0, nblocks]: the segment ranges of the nblocks
+// workgroups, equal shares of the total cost (points + a fixed cost per
+// segment). Recomputed whenever the segments or the workgroup count change.
+hipError_t launch_work_split(const int32_t *seg_start, const int32_t *nseg, int32_t *wg_start, int nblocks,
+                             hipStream_t stream);
 hipError_t launch_p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, GridState gs, int cur,
                       int *err, int n_blocks, hipStream_t stream);
 // G2P including the return mapping of crushed particles. lists: 1 + n ints

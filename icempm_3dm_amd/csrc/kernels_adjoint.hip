@@ -682,7 +682,7 @@ hipError_t launch_adj_g2p_bwd(const KParams &kp, PtsView s0, PtsView s1, PtsView
                               SegInfo seg, const double *V, GridState gb, int cur_b, double *theta, bool plastic,
                               int n_blocks, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
+    n_blocks = segment_grid(n_blocks);
     if (plastic)
         k_adj_g2p_bwd<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, s0, s1, bar, part, seg, V, gb, cur_b, theta);
     else

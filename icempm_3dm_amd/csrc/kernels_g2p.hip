@@ -104,7 +104,7 @@ hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, SegInfo seg, const 
     if (!count_zeroed) k_zero_count<<<1, 1, 0, stream>>>(spill_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
+    n_blocks = segment_grid(n_blocks);
     const PtsView in = src ? *src : pts;
 #define ICEMPM_G2P(E, O, D)                                                                                      \
     k_g2p<E, O, D><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid, record_pq, clk, err, spill,           \

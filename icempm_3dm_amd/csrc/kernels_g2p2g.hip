@@ -141,7 +141,7 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
     if (!count_zeroed) k_zero_count<<<1, 1, 0, stream>>>(spill_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
+    n_blocks = segment_grid(n_blocks);
     const PtsView in = src ? *src : pts;
 #define ICEMPM_G2P2G(E, O, D)                                                                                    \
     k_g2p2g<E, O, D><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,       \

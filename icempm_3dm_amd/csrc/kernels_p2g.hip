@@ -60,14 +60,29 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(KParams kp, PtsView pts, Se
     }
 }
 
+// wg_start[b], b in [0, nblocks]: one thread per workgroup boundary.
+__global__ void k_work_split(const int32_t *seg_start, const int32_t *nseg_p, int32_t *wg_start, int nblocks) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nblocks) return;
+    const int nseg = *nseg_p;
+    const long long total = (long long)seg_start[nseg] + kSegCost * nseg;
+    wg_start[b] = (int32_t)(b == nblocks ? nseg : seg_at_cost(seg_start, nseg, total * b / nblocks));
+}
+
 } // namespace
 
 hipError_t launch_p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, GridState gs, int cur, int *err,
                       int n_blocks, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    n_blocks = (n_blocks + kXcd - 1) / kXcd * kXcd;
+    n_blocks = segment_grid(n_blocks);
     if (kp.ens_n > 0) k_p2g<true><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, gs, cur, err);
     else k_p2g<false><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, gs, cur, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_work_split(const int32_t *seg_start, const int32_t *nseg, int32_t *wg_start, int nblocks,
+                             hipStream_t stream) {
+    k_work_split<<<(nblocks + 1 + 255) / 256, 256, 0, stream>>>(seg_start, nseg, wg_start, nblocks);
     return hipGetLastError();
 }
 

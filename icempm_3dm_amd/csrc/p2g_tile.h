@@ -380,28 +380,31 @@ __device__ __forceinline__ void cursor_load_segment(ChunkCursor &c, const SegInf
 // kSegCost for the tile fetch, clear and flush (512 measured best among
 // 128-1024 on both the 16M-point block and the 64-scene ensemble; an equal
 // number of segments per workgroup was 2-5% slower, an equal number of
-// points 25% slower on the block).
+// points 25% slower on the block). The split is computed once per change
+// of the segments or of the grid (k_work_split), not by every workgroup: a
+// search per workgroup at kernel start adds ~10 us of dependent loads.
 constexpr long long kSegCost = 512;
 
 // First segment s in [0, nseg] whose cumulative cost seg_start[s] + kSegCost s
 // reaches target (seg_start is non-decreasing).
-__device__ __forceinline__ long long seg_at_cost(const SegInfo &seg, int nseg, long long target) {
+__device__ __forceinline__ long long seg_at_cost(const int32_t *seg_start, int nseg, long long target) {
     long long lo = 0, hi = nseg;
     while (lo < hi) {
         const long long mid = (lo + hi) >> 1;
-        if ((long long)seg.seg_start[mid] + kSegCost * mid < target) lo = mid + 1;
+        if ((long long)seg_start[mid] + kSegCost * mid < target) lo = mid + 1;
         else hi = mid;
     }
     return lo;
 }
 
-// The contiguous range of segments of workgroup lb of nblocks: equal shares
-// of the total cost.
+// The contiguous range of segments of workgroup lb (equal shares of the total
+// cost, from the split launch_work_split computed for this grid).
 __device__ __forceinline__ ChunkCursor cursor_first(const SegInfo &seg, int nseg, int lb, int nblocks) {
+    (void)nseg;
+    (void)nblocks;
     ChunkCursor c;
-    const long long total = (long long)seg.seg_start[nseg] + kSegCost * nseg;
-    c.s = seg_at_cost(seg, nseg, total * lb / nblocks);
-    c.s_hi = seg_at_cost(seg, nseg, total * (lb + 1) / nblocks);
+    c.s = seg.wg_start[lb];
+    c.s_hi = seg.wg_start[lb + 1];
     c.begin = c.end = c.c0 = c.brick = 0;
     cursor_load_segment(c, seg);
     return c;
