@@ -1293,7 +1293,7 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     rc = ICEMPM_OK;
     for (int fuse = 1; fuse >= 0 && rc == ICEMPM_OK; fuse--)
         for (int period : {10, 20, 50, 100, 200})
-            for (int bpc : {4, 8, 16}) {
+            for (int bpc : {8, 16, 32}) {
                 if (rc != ICEMPM_OK) break;
                 ctx->fuse = fuse != 0;
                 ctx->sort_period = period;

@@ -329,8 +329,8 @@ int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long l
 
 /* Autotuner (no reference counterpart). Times icempm_step for each
  * combination of G2P+P2G fusion (on/off), brick re-sort period (10, 20, 50,
- * 100, 200 substeps) and workgroups per CU of the brick-segment kernels (4, 8,
- * 16): trial_substeps rounded up to whole periods, each period opening with
+ * 100, 200 substeps) and workgroups per CU of the brick-segment kernels (8, 16,
+ * 32): trial_substeps rounded up to whole periods, each period opening with
  * its re-sort, so the sort is paid in every trial; each from the same device
  * state,
  * then restores the points, the indenter accumulators and the error
