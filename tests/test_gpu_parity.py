@@ -472,13 +472,16 @@ def test_long_run_force_curve_matches_oracle(require_gpu):
     g.close()
 
 
-def test_free_flight_conserves_momentum(require_gpu):
-    """No gravity, the indenter outside the grid and the block clear of the
-    walls: with APIC transfers the total momentum of the points is unchanged
-    by a step (P2G keeps it, the stress and affine terms sum to zero by the
-    first moment of the weights, G2P gives it back) — to round-off, over 40
-    substeps with random velocities, affine fields and deformations."""
-    sc = replace(SMALL, dt=2e-6, gravity=0.0, cells=(12, 10, 12))
+@pytest.mark.parametrize("gravity", [0.0, 9.81])
+def test_free_flight_conserves_momentum(require_gpu, gravity):
+    """The indenter outside the grid and the block clear of the walls: with
+    APIC transfers the total momentum of the points changes by a step only by
+    the gravity impulse -g dt M e_y (P2G keeps it, the stress and affine terms
+    sum to zero by the first moment of the weights, the grid update adds
+    -g dt to every node velocity, G2P gives the grid momentum back) — to
+    round-off, over 40 substeps with random velocities, affine fields and
+    deformations."""
+    sc = replace(SMALL, dt=2e-6, gravity=gravity, cells=(12, 10, 12))
     p = scenes.make_params(sc)
     hs = scenes.perturbed_points(sc, seed=9, amp_F=2e-3, amp_v=0.5, amp_B=20.0, crushed_fraction=0.0)
     L = hs.lines()
@@ -489,9 +492,11 @@ def test_free_flight_conserves_momentum(require_gpu):
     m = p.ParticleMass
     mom0 = m * L[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
     scale = m * np.abs(L[capi.VELX:capi.VELX + 3, :n]).sum(axis=1)
-    g.step(40, 0.0, 0)
+    nsub = 40
+    g.step(nsub, 0.0, 0)
     back = _download(g)
     mom1 = m * back.lines()[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
     assert back.lines()[capi.POSX + 1, :n].min() > 3 * p.cellsize
-    assert np.all(np.abs(mom1 - mom0) <= 1e-12 * scale), (mom0, mom1, scale)
+    impulse = np.array([0.0, -p.Gravity * p.InitialTimeStep * nsub * m * n, 0.0])
+    assert np.all(np.abs(mom1 - (mom0 + impulse)) <= 1e-12 * scale), (mom0, mom1, impulse, scale)
     g.close()

@@ -14,10 +14,12 @@ wrong sign, wrong index or transposed operand fails at least one check:
   g2p          constant and linear grid velocity fields are reproduced exactly
                (v_p = A x_p + b, C = A); F update = (I + dt C) F
   return map   tension branch: det F = J_new, p = -p_t; shear branch:
-               q(F_new; J_tr) = q_{n+1}; U diag V^T invariant to the SVD used
+               q(F_new; J_tr) = q_{n+1}; U diag V^T invariant to the SVD used;
+               a zero deviator (q_{n+1} = 0) leaves F unchanged, not NaN
   grid update  free-fall, clamp, sticky band, walls, indenter contact
   sort         numpy stable argsort; keys of lattice points
-  step         momentum conserved in free flight; mass exact
+  step         momentum changed by the gravity impulse alone in free flight;
+               mass exact
 """
 from __future__ import annotations
 
@@ -266,6 +268,27 @@ def test_return_map_shear_branch():
     assert jp == 1.0
 
 
+@pytest.mark.parametrize("rotated", [False, True])
+def test_return_map_zero_deviator(rotated):
+    """F = s R with p above the compressive strength: q_{n+1} = 0 and the
+    point projects with a trial deviator s_hat of zero. For R = I and
+    s = 63/64 (s^2, 3 s^2 and 3 s^2 / 3 all exact) s_hat is exactly zero and
+    the projection meets Eigen's normalized() of a zero vector
+    (mpm_kernels.cuh:538), which returns it unchanged; a 0/0 there would give
+    NaN. b_hat is then the mean of sigma^2 and F comes back as it went in,
+    finite, with Jp_inv untouched."""
+    p = small_params()
+    R = rotation(RNG) if rotated else np.eye(3)
+    s = 0.99 if rotated else 63 / 64
+    F = s * R
+    ptr, qtr, _ = p_q(p, F)
+    assert ptr > p.IceCompressiveStrength and qtr < 1e-6 * ptr
+    Fn, jp, util, P, Q = _g2p_single(p, F, capi.STATUS_CRUSHED, 1.0)
+    assert np.all(np.isfinite(Fn))
+    assert np.allclose(Fn, F, rtol=0, atol=1e-14)
+    assert jp == 1.0 and util & capi.STATUS_CRUSHED
+
+
 def test_return_map_elastic_point_is_untouched_and_crush_test_sets_bit():
     p = small_params()
     F = np.eye(3) + 1e-7 * RNG.standard_normal((3, 3))
@@ -366,10 +389,12 @@ def test_keys_of_lattice_points():
 
 
 # ------------------------------------------------------------------ whole steps
-def test_free_flight_conserves_momentum_and_mass():
-    """No gravity, no indenter, block away from every wall: total mass and
-    momentum are invariants of P2G -> update -> G2P (APIC)."""
-    sc = replace(scenes.BLOCK_100K, cells=(10, 8, 10), gravity=0.0, dt=2e-6)
+@pytest.mark.parametrize("gravity", [0.0, 9.81])
+def test_free_flight_conserves_momentum_and_mass(gravity):
+    """No indenter, block away from every wall: total mass is an invariant of
+    P2G -> update -> G2P (APIC) and the total momentum changes per substep by
+    the gravity impulse -g dt M e_y alone."""
+    sc = replace(scenes.BLOCK_100K, cells=(10, 8, 10), gravity=gravity, dt=2e-6)
     p = scenes.make_params(sc)
     hs = scenes.make_points(sc)
     n = hs.size
@@ -380,11 +405,13 @@ def test_free_flight_conserves_momentum_and_mass():
     mom0 = (L[capi.VELX:capi.VELX + 3, :n] * p.ParticleMass).sum(axis=1)
     grid = orc.grid_zeros(p)
     bins = np.zeros(p.IndenterArraySize())
-    code, _ = orc.step(p, far, hs.host_buffer, hs.capacity, n, grid, bins, 5, 0.0, 0)
+    nsub = 5
+    code, _ = orc.step(p, far, hs.host_buffer, hs.capacity, n, grid, bins, nsub, 0.0, 0)
     assert code == 0
     assert grid[0].sum() == pytest.approx(n * p.ParticleMass, rel=1e-13)
     mom = (L[capi.VELX:capi.VELX + 3, :n] * p.ParticleMass).sum(axis=1)
-    assert np.allclose(mom, mom0, rtol=0, atol=1e-12 * np.abs(mom0).max())
+    impulse = np.array([0.0, -gravity * p.InitialTimeStep * nsub * n * p.ParticleMass, 0.0])
+    assert np.allclose(mom, mom0 + impulse, rtol=0, atol=1e-12 * np.abs(mom0).max())
 
 
 def test_roundoff_amplification_at_configured_dt():
