@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <utility>
 #include <vector>
@@ -152,7 +153,12 @@ inline void parse_attribute(const File &f, uint64_t at, Dataset &ds) {
     const std::vector<uint64_t> dims = parse_dataspace(f, p);
     p += pad(ssz);
     uint64_t count = 1;
-    for (uint64_t d : dims) count *= d;
+    for (uint64_t d : dims) {
+        if (d != 0 && count > f.d.size() / d) bad("attribute larger than the file");
+        count *= d;
+    }
+    if (a.type.size < 0 || (a.type.size > 0 && count > f.d.size() / (uint64_t)a.type.size))
+        bad("attribute larger than the file");
     const uint64_t nbytes = count * (uint64_t)a.type.size;
     f.need(p, nbytes);
     a.data.assign(f.d.begin() + (long)p, f.d.begin() + (long)(p + nbytes));
@@ -308,10 +314,12 @@ inline std::vector<uint8_t> read_1d(const File &f, const Dataset &ds) {
     if (ds.type.size != 8 || ds.type.big_endian || (ds.type.cls != 0 && ds.type.cls != 1))
         bad("dataset elements are not 8-byte little-endian numbers");
     const uint64_t n = ds.dims[0];
+    // 2^37 elements (1 TiB) bounds the byte count well inside 64 bits
+    if (n > (1ull << 37)) bad("dataset too large");
+    if (ds.layout_class == 1 && ds.addr != kUndef) f.need(ds.addr, n * 8);
     std::vector<uint8_t> out(n * 8, 0);
     if (ds.layout_class == 1) {
         if (ds.addr == kUndef) return out; // never written: fill value 0
-        f.need(ds.addr, n * 8);
         std::memcpy(out.data(), &f.d[ds.addr], n * 8);
     } else if (ds.layout_class == 2) {
         if (ds.chunk.size() != 2 || ds.chunk[1] != 8) bad("unexpected chunk shape");

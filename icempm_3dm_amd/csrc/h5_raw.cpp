@@ -61,6 +61,10 @@ extern "C" int icempm_read_raw_points(const char *path, double *hssoa, unsigned 
         return set_error(ICEMPM_ERR_INVALID_ARG, std::string("read_raw_points: ") + path + ": " + e.msg);
     } catch (const std::bad_alloc &) {
         return set_error(ICEMPM_ERR_CAPACITY, "read_raw_points: out of host memory");
+    } catch (const std::exception &e) {
+        // no exception may cross the C boundary (std::length_error of a size
+        // taken from a damaged file, for one)
+        return set_error(ICEMPM_ERR_INVALID_ARG, std::string("read_raw_points: ") + e.what());
     }
 }
 

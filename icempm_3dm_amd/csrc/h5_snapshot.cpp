@@ -444,6 +444,10 @@ extern "C" int icempm_save_snapshot(const char *path, const icempm_SimParams3D *
         return ICEMPM_OK;
     } catch (const std::bad_alloc &) {
         return set_error(ICEMPM_ERR_CAPACITY, "save_snapshot: out of host memory");
+    } catch (const std::exception &e) {
+        // no exception may cross the C boundary (std::length_error of a size
+        // taken from a damaged file, for one)
+        return set_error(ICEMPM_ERR_INVALID_ARG, std::string("save_snapshot: ") + e.what());
     }
 }
 
@@ -454,11 +458,15 @@ extern "C" int icempm_read_snapshot(const char *path, icempm_SimParams3D *prms, 
     try {
         const Root root = open_root(path);
         const Dataset pts = root.dataset("Points");
-        double v = 0;
-        if (!attr_number(pts, "HSSOA_capacity", v)) bad("Points has no HSSOA_capacity");
-        const uint64_t file_cap = (uint64_t)v;
-        if (!attr_number(pts, "HSSOA_size", v)) bad("Points has no HSSOA_size");
-        const uint64_t file_size = (uint64_t)v;
+        // counts are whole numbers in [0, 2^32); checked before conversion
+        const auto count_attr = [&](const char *name) -> uint64_t {
+            double c = 0;
+            if (!attr_number(pts, name, c)) bad(std::string("Points has no ") + name);
+            if (!(c >= 0 && c <= 4294967295.0)) bad(std::string("Points attribute ") + name + " out of range");
+            return (uint64_t)c;
+        };
+        const uint64_t file_cap = count_attr("HSSOA_capacity");
+        const uint64_t file_size = count_attr("HSSOA_size");
         if (file_size > file_cap || file_cap > 0xffffffffull) bad("inconsistent HSSOA_capacity / HSSOA_size");
         if (pts.dims.size() != 1 || pts.dims[0] != file_cap * ICEMPM_N_PTS_ARRAYS)
             bad("Points does not hold capacity x 28 values");
@@ -480,6 +488,8 @@ extern "C" int icempm_read_snapshot(const char *path, icempm_SimParams3D *prms, 
                 uint8_t *dst = reinterpret_cast<uint8_t *>(prms) + pa.offset;
                 if (pa.kind == 0) std::memcpy(dst, &x, 8);
                 else {
+                    if (!(x >= -2147483648.0 && x <= 2147483647.0))
+                        bad(std::string("Points attribute ") + pa.name + " is not a 32-bit integer");
                     const int32_t iv = (int32_t)x;
                     std::memcpy(dst, &iv, 4);
                 }
@@ -511,5 +521,9 @@ extern "C" int icempm_read_snapshot(const char *path, icempm_SimParams3D *prms, 
         return set_error(ICEMPM_ERR_INVALID_ARG, std::string("read_snapshot: ") + path + ": " + e.msg);
     } catch (const std::bad_alloc &) {
         return set_error(ICEMPM_ERR_CAPACITY, "read_snapshot: out of host memory");
+    } catch (const std::exception &e) {
+        // no exception may cross the C boundary (std::length_error of a size
+        // taken from a damaged file, for one)
+        return set_error(ICEMPM_ERR_INVALID_ARG, std::string("read_snapshot: ") + e.what());
     }
 }

@@ -7,6 +7,20 @@
 
 #include "icempm_3dm.h"
 
+namespace {
+
+// Truncation toward zero like a plain (int) cast, saturated at the int range
+// (NaN gives 0), so that parameters read from a damaged file cannot make the
+// conversion undefined. Equal to the plain cast for every in-range value.
+int sat_int(double v) {
+    if (!(v == v)) return 0;
+    if (v >= 2147483647.0) return INT32_MAX;
+    if (v <= -2147483648.0) return INT32_MIN;
+    return (int)v;
+}
+
+} // namespace
+
 extern "C" {
 
 // SimParams3D::ComputeLame (parameters_sim_3d.cpp:164-169)
@@ -26,7 +40,7 @@ void icempm_params_compute_camclay(icempm_SimParams3D *p) {
 void icempm_params_compute_helper_variables(icempm_SimParams3D *p) {
     p->ParticleMass = p->ParticleVolume * p->Density;
     p->LiquidParticleMass = p->ParticleVolume * p->WaterDensity;
-    p->UpdateEveryNthStep = (int)(1.f / (200 * p->InitialTimeStep));
+    p->UpdateEveryNthStep = sat_int(1.f / (200 * p->InitialTimeStep));
     p->cellsize = p->DomainDimensionX / p->GridXTotal;
     p->cellsize_inv = 1. / p->cellsize;
     p->Dp_inv = 4. / (p->cellsize * p->cellsize);
@@ -36,9 +50,11 @@ void icempm_params_compute_helper_variables(icempm_SimParams3D *p) {
     p->vmax = 0.5 * p->cellsize / p->InitialTimeStep;
     p->vmax_squared = p->vmax * p->vmax;
     p->VectorCapacity_transfer =
-        p->nPartitions > 0 ? (int)(p->nPtsTotal / p->nPartitions * p->PointsTransferBufferFraction) : 0;
-    p->gbOffset = p->GridZ * p->GridY * p->GridHaloSize;
-    p->IndenterSubdivisions = (int)(ICEMPM_PI * p->IndDiameter / p->cellsize);
+        p->nPartitions > 0 ? sat_int(p->nPtsTotal / p->nPartitions * p->PointsTransferBufferFraction) : 0;
+    // the reference's int product; formed in 64 bits so that absurd grid
+    // sizes saturate instead of overflowing
+    p->gbOffset = sat_int((double)((long long)p->GridZ * p->GridY * p->GridHaloSize));
+    p->IndenterSubdivisions = sat_int(ICEMPM_PI * p->IndDiameter / p->cellsize);
 }
 
 // SimParams3D::SetPhi (parameters_sim_3d.h:100)

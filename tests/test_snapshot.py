@@ -151,6 +151,27 @@ def test_errors(tmp_path):
     assert rc == capi.ERR_INVALID_ARG  # no "Points" dataset / truncated
 
 
+def test_derived_parameters_saturate_for_absurd_grids(tmp_path):
+    """A file's grid sizes go into the reference's int product GridZ * GridY *
+    GridHaloSize (gbOffset) and into float-to-int conversions; values no real
+    grid has (here 2^30 x 2^10 nodes) saturate at the int range instead of
+    overflowing, and the rest of the file reads as written. (A host-only
+    AddressSanitizer / UBSan build of the reader also took thousands of
+    randomly damaged files without a report.)"""
+    hs, p, bins = _state(capacity=100, size=50)
+    p.GridY = 1 << 30
+    p.GridZ = 1 << 10
+    p.GridHaloSize = 15
+    f = tmp_path / "d00004.h5"
+    _save(f, p, hs, np.zeros(10), False)
+    cap, size = C.c_uint(0), C.c_uint(0)
+    q = capi.SimParams3D()
+    check(lib().icempm_read_snapshot(os.fsencode(str(f)), C.byref(q), None, 0, C.byref(cap), C.byref(size), None, 0))
+    assert (q.GridY, q.GridZ, q.GridHaloSize) == (1 << 30, 1 << 10, 15)
+    assert q.gbOffset == 2**31 - 1
+    assert (cap.value, size.value) == (100, 50)
+
+
 @pytest.mark.skipif(not RAW.exists(), reason="reference input cloud not mounted here")
 def test_fixed_structures_match_a_libhdf5_file(tmp_path):
     hs, p, bins = _state(capacity=100, size=50)
