@@ -21,12 +21,21 @@ __device__ __forceinline__ void reset_bricks(const KParams &kp, const GridState 
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int32_t *list = gs.brick_list[cur];
-    for (int k = blockIdx.x * kWavesPerBlock + wave; k < count; k += gridDim.x * kWavesPerBlock) {
-        int b = list[k];
-        long long idx = (long long)b * kBrickNodes + lane;
+    const int stride = gridDim.x * kWavesPerBlock;
+    // list entries k0, k0 + stride, ... of the wave, up to 64 at a time, one
+    // per lane (taken by readlane): the loop over bricks then only stores
+    for (int k0 = blockIdx.x * kWavesPerBlock + wave; k0 < count; k0 += 64 * stride) {
+        const int kl = k0 + lane * stride;
+        const int mine = kl < count ? list[kl] : 0;
+        asm volatile("" ::"v"(mine));
+        const int nbatch = min(64, (count - k0 + stride - 1) / stride);
+        for (int j = 0; j < nbatch; j++) {
+            const int b = __builtin_amdgcn_readlane(mine, j);
+            const long long idx = (long long)b * kBrickNodes + lane;
 #pragma unroll
-        for (int l = 0; l < 4; l++) gs.grid[l * kp.grid_pitch + idx] = 0.0;
-        if (lane == 0) gs.brick_flag[b] = 0;
+            for (int l = 0; l < 4; l++) gs.grid[l * kp.grid_pitch + idx] = 0.0;
+            if (lane == 0) gs.brick_flag[b] = 0;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) gs.brick_count[cur ^ 1] = 0;
 }
@@ -60,8 +69,8 @@ __device__ __forceinline__ double law_center(double c0, double t, double v) {
 }
 
 // partition_kernel_update_nodes (mpm_kernels.cuh:138-232) for the nodes of
-// the bricks in list `cur`, one wave per brick, grid-stride; the indenter is
-// the reference cylinder along z or a sphere.
+// the bricks in list `cur`, one wave per brick at a time, grid-stride over
+// the list; the indenter is the reference cylinder along z or a sphere.
 template <bool kEns>
 __device__ __forceinline__ void update_bricks(const KParams &kp0, const GridState &gs, int cur, double cx0, double cy0,
                                               double cz0, double t, double *bins0) {
@@ -69,93 +78,109 @@ __device__ __forceinline__ void update_bricks(const KParams &kp0, const GridStat
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int32_t *list = gs.brick_list[cur];
-    for (int k = blockIdx.x * kWavesPerBlock + wave; k < count; k += gridDim.x * kWavesPerBlock) {
-        const int b = list[k];
-        const long long idx = (long long)b * kBrickNodes + lane;
-        const double mass = gs.grid[idx];
-        if (mass == 0) continue;
-        const KParams kp = scene_view<kEns>(kp0, b);
-        const double dt = kp.dt;
-        const double cellsize = kp.h;
-        const bool sphere = kp.ind_shape == ICEMPM_INDENTER_SPHERE;
-        double cx = cx0, cy = cy0, cz = cz0;
-        if constexpr (kEns) {
-            cx = kp.ind_c0[0] + t * kp.ind_vel[0];
-            cy = kp.ind_c0[1] + t * kp.ind_vel[1];
-            cz = kp.ind_c0[2] + t * kp.ind_vel[2];
-        }
-        double *bins = bins0 + kp.bins_off;
-        const int bz = b % kp.nbz;
-        const int by = (b / kp.nbz) % kp.nby;
-        const int bx = b / (kp.nbz * kp.nby);
-        const int gs_[3] = {bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
-        const double gnpos[3] = {gs_[0] * cellsize, gs_[1] * cellsize, gs_[2] * cellsize};
-        // node index within the scene's grid (walls, sticky band, force bins)
-        const int gi[3] = {gs_[0] - kp.ox, gs_[1] - kp.oy, gs_[2] - kp.oz};
+    const int stride = gridDim.x * kWavesPerBlock;
+    // The wave's bricks are list entries k0, k0 + stride, ...; up to 64 of
+    // them are read at once, one per lane, and taken by readlane, and a
+    // node's mass and momentum are requested together: one memory round trip
+    // per brick instead of three (list entry, mass, momentum).
+    for (int k0 = blockIdx.x * kWavesPerBlock + wave; k0 < count; k0 += 64 * stride) {
+        const int kl = k0 + lane * stride;
+        const int mine = kl < count ? list[kl] : 0;
+        // waited for here, once, not at the head of every brick below (where
+        // the wait would also cover the previous brick's stores)
+        asm volatile("" ::"v"(mine));
+        const int nbatch = min(64, (count - k0 + stride - 1) / stride);
+        for (int j = 0; j < nbatch; j++) {
+            const int b = __builtin_amdgcn_readlane(mine, j);
+            const long long idx = (long long)b * kBrickNodes + lane;
+            const double mass = gs.grid[idx];
+            double v[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) v[i] = gs.grid[(i + 1) * kp0.grid_pitch + idx];
+            // keeps the momentum loads ahead of the branch below, where the
+            // compiler would otherwise move them
+            asm volatile("" ::"v"(v[0]), "v"(v[1]), "v"(v[2]));
+            if (mass == 0) continue;
+            const KParams kp = scene_view<kEns>(kp0, b);
+            const double dt = kp.dt;
+            const double cellsize = kp.h;
+            const bool sphere = kp.ind_shape == ICEMPM_INDENTER_SPHERE;
+            double cx = cx0, cy = cy0, cz = cz0;
+            if constexpr (kEns) {
+                cx = kp.ind_c0[0] + t * kp.ind_vel[0];
+                cy = kp.ind_c0[1] + t * kp.ind_vel[1];
+                cz = kp.ind_c0[2] + t * kp.ind_vel[2];
+            }
+            double *bins = bins0 + kp.bins_off;
+            const int bz = b % kp.nbz;
+            const int by = (b / kp.nbz) % kp.nby;
+            const int bx = b / (kp.nbz * kp.nby);
+            const int gs_[3] = {bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
+            const double gnpos[3] = {gs_[0] * cellsize, gs_[1] * cellsize, gs_[2] * cellsize};
+            // node index within the scene's grid (walls, sticky band, force bins)
+            const int gi[3] = {gs_[0] - kp.ox, gs_[1] - kp.oy, gs_[2] - kp.oz};
 
-        double v[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) v[i] = gs.grid[(i + 1) * kp.grid_pitch + idx];
+            for (int i = 0; i < 3; i++) v[i] /= mass;
+            v[1] -= kp.dt_gravity;
+            double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (vsq > kp.vmax_sq) {
+                double nrm = sqrt(vsq);
 #pragma unroll
-        for (int i = 0; i < 3; i++) v[i] /= mass;
-        v[1] -= kp.dt_gravity;
-        double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        if (vsq > kp.vmax_sq) {
-            double nrm = sqrt(vsq);
-#pragma unroll
-            for (int i = 0; i < 3; i++) v[i] = v[i] / nrm * kp.vmax;
-        }
+                for (int i = 0; i < 3; i++) v[i] = v[i] / nrm * kp.vmax;
+            }
 
-        if (kp.setup_type != 2) {
-            double n[3] = {gnpos[0] - cx, gnpos[1] - cy, sphere ? gnpos[2] - cz : 0.0};
-            double nsq = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-            if (nsq < kp.ind_rsq) {
-                double vrel[3] = {v[0] - kp.ind_vel[0], v[1] - kp.ind_vel[1], v[2] - kp.ind_vel[2]};
-                if (nsq > 0) {
-                    double nrm = sqrt(nsq);
+            if (kp.setup_type != 2) {
+                double n[3] = {gnpos[0] - cx, gnpos[1] - cy, sphere ? gnpos[2] - cz : 0.0};
+                double nsq = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+                if (nsq < kp.ind_rsq) {
+                    double vrel[3] = {v[0] - kp.ind_vel[0], v[1] - kp.ind_vel[1], v[2] - kp.ind_vel[2]};
+                    if (nsq > 0) {
+                        double nrm = sqrt(nsq);
 #pragma unroll
-                    for (int i = 0; i < 3; i++) n[i] /= nrm;
-                }
-                double vn = vrel[0] * n[0] + vrel[1] * n[1] + vrel[2] * n[2];
-                if (vn < 0) {
-                    double prev[3] = {v[0], v[1], v[2]};
+                        for (int i = 0; i < 3; i++) n[i] /= nrm;
+                    }
+                    double vn = vrel[0] * n[0] + vrel[1] * n[1] + vrel[2] * n[2];
+                    if (vn < 0) {
+                        double prev[3] = {v[0], v[1], v[2]};
 #pragma unroll
-                    for (int i = 0; i < 3; i++) v[i] = kp.ind_vel[i] + (vrel[i] - n[i] * vn);
-                    double force[3];
+                        for (int i = 0; i < 3; i++) v[i] = kp.ind_vel[i] + (vrel[i] - n[i] * vn);
+                        double force[3];
 #pragma unroll
-                    for (int i = 0; i < 3; i++) force[i] = (prev[i] - v[i]) * mass / dt;
-                    // no sensor bins to write when IndenterSubdivisions is 0
-                    if (kp.ind_subdiv > 0) {
-                    float angle = atan2f((float)n[0], (float)n[1]);
-                    angle += ICEMPM_PI;
-                    angle *= kp.ind_subdiv / (2 * ICEMPM_PI);
-                    int index_angle = min(max((int)angle, 0), kp.ind_subdiv - 1);
-                    int index_z = min(max(gi[2], 0), kp.gz - 1);
-                    int index = index_z + index_angle * kp.gz;
+                        for (int i = 0; i < 3; i++) force[i] = (prev[i] - v[i]) * mass / dt;
+                        // no sensor bins to write when IndenterSubdivisions is 0
+                        if (kp.ind_subdiv > 0) {
+                        float angle = atan2f((float)n[0], (float)n[1]);
+                        angle += ICEMPM_PI;
+                        angle *= kp.ind_subdiv / (2 * ICEMPM_PI);
+                        int index_angle = min(max((int)angle, 0), kp.ind_subdiv - 1);
+                        int index_z = min(max(gi[2], 0), kp.gz - 1);
+                        int index = index_z + index_angle * kp.gz;
 #pragma unroll
-                    for (int i = 0; i < 3; i++) unsafeAtomicAdd(&bins[i + 3 * index], force[i]);
+                        for (int i = 0; i < 3; i++) unsafeAtomicAdd(&bins[i + 3 * index], force[i]);
+                        }
                     }
                 }
             }
-        }
 
-        if (kp.setup_type != 2) {
-            if (gi[1] <= 2) v[0] = v[1] = v[2] = 0;
-        } else {
-            if (gi[1] <= 2 && v[1] < 0) v[1] = 0;
-        }
-        if (gi[1] >= kp.gy - 3 && v[1] > 0) v[1] = 0;
-        if (gi[0] <= 2 && v[0] < 0) v[0] = 0;
-        else if (gi[0] >= kp.gx - 3 && v[0] > 0) v[0] = 0;
-        if (gi[2] <= 2 && v[2] < 0) v[2] = 0;
-        else if (gi[2] >= kp.gz - 3 && v[2] > 0) v[2] = 0;
+            if (kp.setup_type != 2) {
+                if (gi[1] <= 2) v[0] = v[1] = v[2] = 0;
+            } else {
+                if (gi[1] <= 2 && v[1] < 0) v[1] = 0;
+            }
+            if (gi[1] >= kp.gy - 3 && v[1] > 0) v[1] = 0;
+            if (gi[0] <= 2 && v[0] < 0) v[0] = 0;
+            else if (gi[0] >= kp.gx - 3 && v[0] > 0) v[0] = 0;
+            if (gi[2] <= 2 && v[2] < 0) v[2] = 0;
+            else if (gi[2] >= kp.gz - 3 && v[2] > 0) v[2] = 0;
 
-        if (kp.setup_type == 2 && kp.drag != 0) {
+            if (kp.setup_type == 2 && kp.drag != 0) {
 #pragma unroll
-            for (int i = 0; i < 3; i++) v[i] *= (1 - kp.drag);
+                for (int i = 0; i < 3; i++) v[i] *= (1 - kp.drag);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) gs.grid[(i + 1) * kp.grid_pitch + idx] = v[i];
         }
-#pragma unroll
-        for (int i = 0; i < 3; i++) gs.grid[(i + 1) * kp.grid_pitch + idx] = v[i];
     }
 }
 
