@@ -472,6 +472,43 @@ def test_long_run_force_curve_matches_oracle(require_gpu):
     g.close()
 
 
+@pytest.mark.parametrize("shape", [capi.INDENTER_SPHERE, capi.INDENTER_CYLINDER_Z])
+def test_indenter_impulse_balances_momentum(require_gpu, shape):
+    """Clear of the walls and below the velocity clamp, the points' total
+    momentum changes over n substeps by the gravity impulse minus dt times
+    the sum of the raw force accumulators, which hold each contact node's
+    (v_before - v_after) m / dt: the device's sensors account for all of the
+    momentum its indenter removes (and agree with the oracle's)."""
+    from test_oracle import _indented_lifted_block
+
+    p, hs, law = _indented_lifted_block(shape, SMALL)
+    n = hs.size
+    m = p.ParticleMass
+    L = hs.lines()
+    mom0 = m * L[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
+    ref = hs.copy()
+    g = _gpu_with(p, hs, indenter=law)
+    g.set_sort_period(3)
+    nsub = 10
+    g.reset_indenter_force_accumulator()
+    g.step(nsub, 0.0, 0)
+    F = g.read_indenter_bins().reshape(-1, 3).sum(axis=0)
+    back = _download(g)
+    mom1 = m * back.lines()[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
+    dt = p.InitialTimeStep
+    gravity = np.array([0.0, -p.Gravity * dt * nsub * m * n, 0.0])
+    scale = m * np.abs(back.lines()[capi.VELX:capi.VELX + 3, :n]).sum() + dt * np.abs(F).sum()
+    assert dt * abs(F[1]) > 10 * abs(gravity[1])
+    assert np.all(np.abs(mom1 - (mom0 + gravity - dt * F)) <= 1e-12 * scale), (mom0, mom1, F, scale)
+    bins = np.zeros(p.IndenterArraySize())
+    code, _ = orc.step(p, capi.Indenter.make(*law), ref.host_buffer, ref.capacity, n, orc.grid_zeros(p), bins, nsub,
+                       0.0, 0, 1)
+    assert code == 0
+    Fr = bins.reshape(-1, 3).sum(axis=0)
+    assert normwise(F, Fr) <= 1e-9, (F, Fr)
+    g.close()
+
+
 @pytest.mark.parametrize("gravity", [0.0, 9.81])
 def test_free_flight_conserves_momentum(require_gpu, gravity):
     """The indenter outside the grid and the block clear of the walls: with

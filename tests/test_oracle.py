@@ -18,7 +18,8 @@ wrong sign, wrong index or transposed operand fails at least one check:
                a zero deviator (q_{n+1} = 0) leaves F unchanged, not NaN
   grid update  free-fall, clamp, sticky band, walls, indenter contact
   sort         numpy stable argsort; keys of lattice points
-  step         momentum changed by the gravity impulse alone in free flight;
+  step         momentum changed by the gravity impulse (and by dt x the
+               indenter force sensors) alone when clear of the walls;
                mass exact
 """
 from __future__ import annotations
@@ -412,6 +413,48 @@ def test_free_flight_conserves_momentum_and_mass(gravity):
     mom = (L[capi.VELX:capi.VELX + 3, :n] * p.ParticleMass).sum(axis=1)
     impulse = np.array([0.0, -gravity * p.InitialTimeStep * nsub * n * p.ParticleMass, 0.0])
     assert np.allclose(mom, mom0 + impulse, rtol=0, atol=1e-12 * np.abs(mom0).max())
+
+
+def _indented_lifted_block(shape, sc_base):
+    """A block lifted 8 cells off the sticky band (every wall and the band
+    stay clear), with the indenter lifted alike and pressed half a cell into
+    its top face, coming down at 5 m/s."""
+    sc = replace(sc_base, cells=(10, 8, 10), dt=2e-6, indenter_shape=shape, indenter_speed=5.0)
+    p = scenes.make_params(sc)
+    hs = scenes.make_points(sc)
+    n = hs.size
+    h = p.cellsize
+    hs.lines()[capi.POSX + 1, :n] += 8 * h
+    shape, c, vel = sc.indenter()
+    return p, hs, (shape, (c[0], c[1] + 8 * h - 0.5 * h, c[2]), vel)
+
+
+@pytest.mark.parametrize("shape", [capi.INDENTER_SPHERE, capi.INDENTER_CYLINDER_Z])
+def test_indenter_impulse_balances_momentum(shape):
+    """Clear of the walls and below the velocity clamp, only gravity and the
+    indenter change the total momentum of the points, and the indenter's
+    share is what its force sensors record: each contact node's
+    (v_before - v_after) m / dt (mpm_kernels.cuh:186-200) is added to a bin,
+    so over n substeps Sum m v changes by -g dt n M e_y - dt Sum(raw bins).
+    A wrong sign, a missing mass or dt, or a contact node that reaches no bin
+    breaks the balance."""
+    p, hs, law = _indented_lifted_block(shape, scenes.BLOCK_100K)
+    ind = capi.Indenter.make(*law)
+    n = hs.size
+    L = hs.lines()
+    m = p.ParticleMass
+    mom0 = m * L[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
+    bins = np.zeros(p.IndenterArraySize())
+    nsub = 10
+    code, _ = orc.step(p, ind, hs.host_buffer, hs.capacity, n, orc.grid_zeros(p), bins, nsub, 0.0, 0)
+    assert code == 0
+    mom1 = m * L[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
+    F = bins.reshape(-1, 3).sum(axis=0)
+    gravity = np.array([0.0, -p.Gravity * p.InitialTimeStep * nsub * m * n, 0.0])
+    expected = mom0 + gravity - p.InitialTimeStep * F
+    scale = m * np.abs(L[capi.VELX:capi.VELX + 3, :n]).sum() + p.InitialTimeStep * np.abs(F).sum()
+    assert p.InitialTimeStep * abs(F[1]) > 10 * abs(gravity[1]), "the indenter must dominate"
+    assert np.all(np.abs(mom1 - expected) <= 1e-12 * scale), (mom1, expected, scale)
 
 
 def test_roundoff_amplification_at_configured_dt():
