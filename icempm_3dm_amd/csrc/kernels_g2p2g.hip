@@ -98,7 +98,19 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         pf.util = ICEMPM_STATUS_DISABLED;
         if (nx.valid() && nx.c0 + t < nx.end) load_g2p_state(in, nx.c0 + t, pf);
         p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
-        if (c.last()) p2g_flush(sh, t, ks, org, bc, gs_b, bl);
+        if (c.last()) {
+            p2g_flush_nodes(sh, t, ks, org, gs_b);
+            __syncthreads();
+            p2g_mark_tile_bricks(sh, t, ks, bc, bl);
+            // One scene needs no barrier here: the next segment's set-up above
+            // writes the velocity tile (last read by the gathers, before the
+            // record barrier) and clears the P2G tile (last read by the flush,
+            // before the barrier above), so only wavefront 0's brick marking
+            // is still running when the other wavefronts start it, and the
+            // set-up's barrier joins them (1.5% faster on a 16M-point block).
+            // The ensemble instantiation measured 1% slower without it.
+            if constexpr (kEns) __syncthreads();
+        }
         c = nx;
     }
 }

@@ -328,10 +328,11 @@ __device__ __forceinline__ void flush_order(int k, int &tx, int &ty, int &tz) {
     tz = piece_start(pz) + iz;
 }
 
-// Adds the tile to the grid and marks the touched bricks (call after a
-// barrier; ends with a barrier).
-__device__ __forceinline__ void p2g_flush(P2GShared &sh, int t, const KParams &kp, const int org[3],
-                                          const int bc[3], const GridState &gs, const BrickList &bl) {
+// Adds the tile to the grid and collects the touched bricks in
+// sh.brickmask (call after a barrier; the mask is complete, and the tile
+// free, after the next barrier).
+__device__ __forceinline__ void p2g_flush_nodes(P2GShared &sh, int t, const KParams &kp, const int org[3],
+                                                const GridState &gs) {
     for (int k = t; k < kTileNodes; k += kP2GThreads) {
         int tx, ty, tz;
         flush_order(k, tx, ty, tz);
@@ -348,12 +349,28 @@ __device__ __forceinline__ void p2g_flush(P2GShared &sh, int t, const KParams &k
             atomicOr(&sh.brickmask, 1u << ((dbx * 3 + dby) * 3 + dbz));
         }
     }
-    __syncthreads();
+}
+
+// Lists the bricks of sh.brickmask (after p2g_flush_nodes and a barrier).
+// Only lanes 0-26, all in wavefront 0, read the mask, so its clear by lane 0
+// (p2g_zero_tile) later in the same wavefront's program needs no barrier in
+// between.
+__device__ __forceinline__ void p2g_mark_tile_bricks(const P2GShared &sh, int t, const KParams &kp, const int bc[3],
+                                                     const BrickList &bl) {
     if (t < 27 && ((sh.brickmask >> t) & 1u)) {
         const int dbx = t / 9, dby = (t / 3) % 3, dbz = t % 3;
         const long long b = ((long long)(bc[0] - 1 + dbx) * kp.nby + (bc[1] - 1 + dby)) * kp.nbz + (bc[2] - 1 + dbz);
         mark_brick(bl, b);
     }
+}
+
+// Adds the tile to the grid and marks the touched bricks (call after a
+// barrier; ends with a barrier).
+__device__ __forceinline__ void p2g_flush(P2GShared &sh, int t, const KParams &kp, const int org[3],
+                                          const int bc[3], const GridState &gs, const BrickList &bl) {
+    p2g_flush_nodes(sh, t, kp, org, gs);
+    __syncthreads();
+    p2g_mark_tile_bricks(sh, t, kp, bc, bl);
     __syncthreads();
 }
 
