@@ -94,10 +94,14 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         // P2G record of the new state, straight from registers
         const double Je = updated ? det3(s.Fe) : 1.0;
         p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rcbrt(Je * Je), s.Bp, s.Jp, org, err);
+        // one scene: the record barrier also publishes the scatter's counts of
+        // points outside the tile, one barrier fewer per chunk (0.4% on a
+        // 16M-point block; the ensemble instantiation measured 1% slower)
+        if constexpr (!kEns) p2g_count_over(sh, t);
         __syncthreads();
         pf.util = ICEMPM_STATUS_DISABLED;
         if (nx.valid() && nx.c0 + t < nx.end) load_g2p_state(in, nx.c0 + t, pf);
-        p2g_scatter(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
+        p2g_scatter<!kEns>(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
         if (c.last()) {
             p2g_flush_nodes(sh, t, ks, org, gs_b);
             __syncthreads();

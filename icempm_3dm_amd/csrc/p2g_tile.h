@@ -221,8 +221,18 @@ __device__ __forceinline__ void column_shift(Column &c) {
     c.v2[2] = 0.0;
 }
 
+// Per wavefront, the number of its points outside the tile (after the
+// thread's own p2g_record; the barrier that completes the records also
+// publishes the counts).
+__device__ __forceinline__ void p2g_count_over(P2GShared &sh, int t) {
+    const unsigned long long om = __ballot(sh.base[t] == -2);
+    if ((t & 63) == 0) sh.wave_count[t >> 6] = __popcll(om);
+}
+
 // Scatter of a sub-chunk whose records are complete (call after a barrier;
-// ends with a barrier). nchunk: number of record slots used.
+// ends with a barrier). nchunk: number of record slots used. kCounted: the
+// caller ran p2g_count_over before that barrier.
+template <bool kCounted = false>
 __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, const KParams &kp,
                                             const GridState &gs, const BrickList &bl) {
     const int lane = t & 63;
@@ -230,8 +240,10 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
     // list of points outside the tile
     const bool over = sh.base[t] == -2;
     const unsigned long long om = __ballot(over);
-    if (lane == 0) sh.wave_count[wave] = __popcll(om);
-    __syncthreads();
+    if constexpr (!kCounted) {
+        if (lane == 0) sh.wave_count[wave] = __popcll(om);
+        __syncthreads();
+    }
     {
         int offo = 0;
         for (int w = 0; w < wave; w++) offo += sh.wave_count[w];
