@@ -240,11 +240,24 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
     // list of points outside the tile
     const bool over = sh.base[t] == -2;
     const unsigned long long om = __ballot(over);
-    if constexpr (!kCounted) {
+    int nover_counted = 0;
+    if constexpr (kCounted) {
+        // every thread totals the published counts; the list and its barrier
+        // are needed only when some point is outside the tile (the same
+        // total everywhere, so the branch is uniform)
+        int offo = 0;
+        for (int w = 0; w < kP2GThreads / 64; w++) {
+            const int c = sh.wave_count[w];
+            nover_counted += c;
+            if (w < wave) offo += c;
+        }
+        if (nover_counted > 0) {
+            if (over) sh.over[offo + __popcll(om & ((1ull << lane) - 1ull))] = t;
+            __syncthreads();
+        }
+    } else {
         if (lane == 0) sh.wave_count[wave] = __popcll(om);
         __syncthreads();
-    }
-    {
         int offo = 0;
         for (int w = 0; w < wave; w++) offo += sh.wave_count[w];
         if (over) sh.over[offo + __popcll(om & ((1ull << lane) - 1ull))] = t;
@@ -253,8 +266,8 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
             for (int w = 0; w < kP2GThreads / 64; w++) to += sh.wave_count[w];
             sh.nover = to;
         }
+        __syncthreads();
     }
-    __syncthreads();
 
     // lane roles: 7 groups of 9 lanes per wave; lane (ox,oy) owns a node column
     const int gq = lane % 9;
@@ -290,7 +303,7 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
         }
         if (cur_pk >= 0) column_to_tile(sh, cur_pk, ox, oy, col);
     }
-    const int nover = sh.nover;
+    const int nover = kCounted ? nover_counted : sh.nover;
     for (int r = group_lane ? group : nover; r < nover; r += kGroups) {
         const int j = sh.over[r];
         const int ab = sh.absb[j];
