@@ -71,7 +71,18 @@ __device__ __forceinline__ KParams scene_of_brick(const KParams &kp, long long b
     const int bx = (int)(b / ((long long)kp.nbz * kp.nby));
     const int slot = ((bx / kp.ens_slot_b[0]) * kp.ens_nslot[1] + by / kp.ens_slot_b[1]) * kp.ens_nslot[2] +
                      bz / kp.ens_slot_b[2];
-    return kp.ens_slots[slot];
+    // The table is written by the host only, so it is read through the
+    // constant address space: with a workgroup-uniform brick the compiler
+    // then uses scalar loads and scalar registers instead of ~40 VGPRs.
+    static_assert(sizeof(KParams) % 8 == 0, "copied in 8-byte words");
+    using ConstWord = __attribute__((address_space(4))) const unsigned long long;
+    ConstWord *src = (ConstWord *)(kp.ens_slots + slot);
+    unsigned long long w[sizeof(KParams) / 8];
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(KParams) / 8); k++) w[k] = src[k];
+    KParams out;
+    __builtin_memcpy(&out, w, sizeof(KParams));
+    return out;
 }
 
 // The scene block for work on brick b: kp itself for one scene.
