@@ -94,26 +94,25 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         // P2G record of the new state, straight from registers
         const double Je = updated ? det3(s.Fe) : 1.0;
         p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rcbrt(Je * Je), s.Bp, s.Jp, org, err);
-        // one scene: the record barrier also publishes the scatter's counts of
-        // points outside the tile, one barrier fewer per chunk (0.4% on a
-        // 16M-point block; the ensemble instantiation measured 1% slower)
-        if constexpr (!kEns) p2g_count_over(sh, t);
+        // the record barrier also publishes the scatter's counts of points
+        // outside the tile: one barrier fewer per chunk, and none for the
+        // out-of-tile list when the counts are all zero
+        p2g_count_over(sh, t);
         __syncthreads();
         pf.util = ICEMPM_STATUS_DISABLED;
         if (nx.valid() && nx.c0 + t < nx.end) load_g2p_state(in, nx.c0 + t, pf);
-        p2g_scatter<!kEns>(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
+        p2g_scatter<true>(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
         if (c.last()) {
             p2g_flush_nodes(sh, t, ks, org, gs_b);
             __syncthreads();
             p2g_mark_tile_bricks(sh, t, ks, bc, bl);
-            // One scene needs no barrier here: the next segment's set-up above
-            // writes the velocity tile (last read by the gathers, before the
-            // record barrier) and clears the P2G tile (last read by the flush,
-            // before the barrier above), so only wavefront 0's brick marking
-            // is still running when the other wavefronts start it, and the
-            // set-up's barrier joins them (1.5% faster on a 16M-point block).
-            // The ensemble instantiation measured 1% slower without it.
-            if constexpr (kEns) __syncthreads();
+            // No barrier here: the next segment's set-up above writes the
+            // velocity tile (last read by the gathers, before the record
+            // barrier) and clears the P2G tile (last read by the flush, before
+            // the barrier above), so only wavefront 0's brick marking is still
+            // running when the other wavefronts start it, and the set-up's
+            // barrier joins them.
+
         }
         c = nx;
     }
