@@ -30,7 +30,7 @@ DEVERR_NONFINITE = 76
 ADJ_UNSUPPORTED = 91
 INDENTER_CYLINDER_Z, INDENTER_SPHERE = 0, 1
 PHASES = ("sort", "reset", "p2g", "update", "g2p", "g2p2g", "graph")
-STATS = ("spilled", "sorts", "segments")
+STATS = ("spilled", "sorts", "segments", "p2g_out_of_tile", "g2p_spilled_total")
 
 
 class SimParams3D(C.Structure):

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -56,11 +57,15 @@ struct TimedLaunch {
 // kernel parameter block, point count, launch sizes, and the host-side
 // ping-pong state (grid, brick lists, point buffer) it starts from. Compared
 // bytewise, so it is zero-filled before the fields are set.
-struct GraphKey {
+struct GraphKeyFields {
     KParams kp;
     int n, p2g_blocks, g2p_blocks, fuse, sort_first, run;
     int g, cur0, cur1, cur_pts, clock_slot;
 };
+// A graph's key is the bytes of a GraphKeyFields whose padding is zero. It is
+// kept and compared as a byte array: copies of a struct need not carry its
+// padding, copies of an array of bytes do.
+using GraphKey = std::array<unsigned char, sizeof(GraphKeyFields)>;
 
 struct GraphEntry {
     GraphKey key;
@@ -118,6 +123,7 @@ struct icempm_ctx {
     double *bins = nullptr;
     long long nbins = 0;
     int *err = nullptr;
+    unsigned long long *stats = nullptr; // kStat* counters (KParams::stats)
     double *grid_xmajor = nullptr; // lazily allocated for download_grid
 
     int sort_period = 0;
@@ -261,10 +267,11 @@ icempm_Indenter reference_indenter(const icempm_SimParams3D &P) {
 }
 
 // Kernel parameters of one scene whose node (0,0,0) sits at node `origin` of a
-// stored grid of nb bricks per axis.
-KParams scene_kparams(const icempm_SimParams3D &P, const icempm_Indenter &ind, const icempm_Damage &dmg,
-                      const int origin[3], const int nb[3]) {
-    KParams k{};
+// stored grid of nb bricks per axis, written into k. Every byte of k is
+// cleared first, padding included: graph keys compare the block bytewise.
+void scene_kparams(const icempm_SimParams3D &P, const icempm_Indenter &ind, const icempm_Damage &dmg,
+                   const int origin[3], const int nb[3], KParams &k) {
+    std::memset(&k, 0, sizeof(KParams));
     k.gx = P.GridXTotal;
     k.gy = P.GridY;
     k.gz = P.GridZ;
@@ -308,7 +315,6 @@ KParams scene_kparams(const icempm_SimParams3D &P, const icempm_Indenter &ind, c
         k.ind_vel[d] = ind.velocity[d];
         k.ind_c0[d] = ind.center_initial[d] + origin[d] * P.cellsize;
     }
-    return k;
 }
 
 // The return mapping's constants of a parameter block (KParams::rmc layout).
@@ -331,7 +337,7 @@ int fill_kparams(icempm_ctx *ctx) {
     int nb[3] = {(P.GridXTotal + 3) / 4, (P.GridY + 3) / 4, (P.GridZ + 3) / 4};
     if (ctx->ens.n > 0)
         for (int d = 0; d < 3; d++) nb[d] = ctx->ens.slot_b[d] * ctx->ens.nslot[d];
-    ctx->kp = scene_kparams(P, ctx->ind, ctx->damage, origin, nb);
+    scene_kparams(P, ctx->ind, ctx->damage, origin, nb, ctx->kp);
     if (ctx->ens.n > 0) {
         ctx->kp.ens_n = ctx->ens.n;
         for (int d = 0; d < 3; d++) {
@@ -341,6 +347,7 @@ int fill_kparams(icempm_ctx *ctx) {
         ctx->kp.ens_slots = ctx->ens.d_slots;
     }
     ctx->kp.rmc = ctx->rmc;
+    ctx->kp.stats = ctx->stats;
     if (ctx->rmc) {
         double v[kRmcSize];
         rmc_values(ctx->kp, v);
@@ -420,6 +427,8 @@ void free_all(icempm_ctx *ctx) {
     f(ctx->work);
     f(ctx->bins);
     f(ctx->err);
+    f(ctx->stats);
+    ctx->stats = nullptr;
     f(ctx->grid_xmajor);
     ctx->scratch = ctx->seg_start = ctx->seg_brick = ctx->nseg = ctx->work = nullptr;
     ctx->bins = nullptr;
@@ -571,7 +580,7 @@ int step_run(icempm_ctx *ctx, int run, double &t, long long step0, bool clocked)
 }
 
 GraphKey graph_key(const icempm_ctx *ctx, bool sort_first, int run) {
-    GraphKey k;
+    GraphKeyFields k;
     std::memset(&k, 0, sizeof(k));
     std::memcpy(&k.kp, &ctx->kp, sizeof(KParams));
     k.n = ctx->n;
@@ -585,7 +594,9 @@ GraphKey graph_key(const icempm_ctx *ctx, bool sort_first, int run) {
     k.cur1 = ctx->grids[1].cur;
     k.cur_pts = ctx->cur_pts;
     k.clock_slot = ctx->clock_slot;
-    return k;
+    GraphKey bytes;
+    std::memcpy(bytes.data(), &k, sizeof(k));
+    return bytes;
 }
 
 // Captures [brick sort +] one run of substeps on the clock into an
@@ -635,7 +646,7 @@ int capture_run(icempm_ctx *ctx, const GraphKey &key, bool sort_first, int run, 
 
 GraphEntry *find_graph(icempm_ctx *ctx, const GraphKey &key) {
     for (auto &e : ctx->graphs)
-        if (std::memcmp(&e.key, &key, sizeof(GraphKey)) == 0) return &e;
+        if (e.key == key) return &e;
     return nullptr;
 }
 
@@ -734,6 +745,9 @@ int check_grid(const icempm_SimParams3D *prms) {
 int alloc_common(icempm_ctx *ctx, const icempm_SimParams3D &prms, int n_points_capacity, long long nbins) {
     ctx->prms = prms;
     ICEMPM_TRY(dev_alloc(&ctx->rmc, kRmcSize, "return-mapping constants"));
+    ICEMPM_TRY(dev_alloc(&ctx->stats, kStatCount, "step counters"));
+    k_fill_double<<<1, 64, 0, ctx->stream>>>((double *)ctx->stats, kStatCount, 0.0); // all-zero bits
+    ICEMPM_HIP(hipGetLastError());
     ICEMPM_TRY(fill_kparams(ctx));
     const long long nb = (long long)ctx->kp.nbx * ctx->kp.nby * ctx->kp.nbz;
     if (nb * kBrickNodes > (1ll << 40) || nb > 0x7FFFFFF)
@@ -810,7 +824,8 @@ int ens_upload_slots(icempm_ctx *ctx) {
     std::vector<double> rmc((size_t)nslots * kRmcSize);
     for (int k = 0; k < nslots; k++) {
         const int sc = k < E.n ? k : 0;
-        KParams e = scene_kparams(E.prms[sc], E.ind[sc], ctx->damage, &E.origin[3 * sc], nb);
+        KParams &e = table[(size_t)k];
+        scene_kparams(E.prms[sc], E.ind[sc], ctx->damage, &E.origin[3 * sc], nb, e);
         rmc_values(e, &rmc[(size_t)k * kRmcSize]);
         e.rmc = E.d_rmc + (size_t)k * kRmcSize;
         e.bins_off = E.bins_off[sc];
@@ -820,7 +835,7 @@ int ens_upload_slots(icempm_ctx *ctx) {
             e.ens_nslot[d] = E.nslot[d];
         }
         e.ens_slots = E.d_slots;
-        table[k] = e;
+        e.stats = ctx->stats;
     }
     ICEMPM_HIP(hipMemcpyAsync(E.d_slots, table.data(), sizeof(KParams) * nslots, hipMemcpyHostToDevice,
                               ctx->stream));
@@ -1360,10 +1375,14 @@ int icempm_read_stats(icempm_ctx *ctx, long long *out, int n_out) {
     ICEMPM_TRY(check_ready(ctx));
     if (!out || n_out < 0) return fail(ICEMPM_ERR_INVALID_ARG, "bad stats buffer");
     int dev[2] = {0, 0};
+    unsigned long long cum[kStatCount] = {0, 0};
     ICEMPM_HIP(hipMemcpyAsync(&dev[0], ctx->work, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ICEMPM_HIP(hipMemcpyAsync(&dev[1], ctx->nseg, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ICEMPM_HIP(hipMemcpyAsync(cum, ctx->stats, sizeof(cum), hipMemcpyDeviceToHost, ctx->stream));
     ICEMPM_HIP(hipStreamSynchronize(ctx->stream));
-    const long long v[ICEMPM_N_STATS] = {dev[0], ctx->n_sorts, dev[1]};
+    static_assert(ICEMPM_N_STATS == 5, "stats layout");
+    const long long v[ICEMPM_N_STATS] = {dev[0], ctx->n_sorts, dev[1], (long long)cum[kStatP2GOutOfTile],
+                                         (long long)cum[kStatG2PSpilled]};
     for (int i = 0; i < n_out && i < ICEMPM_N_STATS; i++) out[i] = v[i];
     return ICEMPM_OK;
 }

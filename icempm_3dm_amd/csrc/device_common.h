@@ -43,6 +43,9 @@ struct KParams {
     // as kernel arguments they would be held in scalar registers for the
     // whole fused kernel, which then spills.
     const double *rmc;
+    // Step counters of the context (kStat* below), added to by atomics on the
+    // rare paths they count; nullptr: not counted.
+    unsigned long long *stats;
 
     int ind_shape, ind_subdiv;
     double ind_rsq;
@@ -59,6 +62,17 @@ struct KParams {
     int ens_nslot[3];  // slots per axis
     const KParams *ens_slots;
 };
+
+// Layout of KParams::stats: points that a P2G scatter (plain, fused or the
+// adjoint's) found outside its segment's tile and sent straight to HBM, and
+// points that a G2P found outside the tile and took the spill path; both
+// summed since allocation.
+constexpr int kStatP2GOutOfTile = 0, kStatG2PSpilled = 1;
+constexpr int kStatCount = 2;
+
+__device__ __forceinline__ void count_stat(const KParams &kp, int which, unsigned long long n) {
+    if (kp.stats && n) atomicAdd(kp.stats + which, n);
+}
 
 // Layout of KParams::rmc.
 constexpr int kRmcThresholdP = 0, kRmcTanPhi = 1, kRmcJeNew = 2, kRmcCbrtJeNew = 3, kRmcDmgRate = 4, kRmcDmgSoft = 5;

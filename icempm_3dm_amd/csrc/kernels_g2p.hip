@@ -78,6 +78,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p_spill(KParams kp, PtsView p
                                                            const int *spill_count, PtsView in) {
     if (clk) record_pq = clk->record;
     const int count = *spill_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) count_stat(kp, kStatG2PSpilled, (unsigned long long)count);
     const double c4h = 4. * kp.h_inv;
     for (int k = blockIdx.x * kP2GThreads + threadIdx.x; k < count; k += gridDim.x * kP2GThreads) {
         const int i = spill[k];

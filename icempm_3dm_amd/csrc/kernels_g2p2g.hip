@@ -17,6 +17,12 @@
 // (k_g2p2g_spill).
 #include "g2p_point.h"
 
+// 1: a barrier after the brick marking at every segment end (the kernel is
+// correct without it; the switch exists to compare the two builds).
+#ifndef ICEMPM_SEGMENT_END_BARRIER
+#define ICEMPM_SEGMENT_END_BARRIER 0
+#endif
+
 namespace icempm {
 
 namespace {
@@ -112,7 +118,9 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
             // the barrier above), so only wavefront 0's brick marking is still
             // running when the other wavefronts start it, and the set-up's
             // barrier joins them.
-
+#if ICEMPM_SEGMENT_END_BARRIER
+            __syncthreads();
+#endif
         }
         c = nx;
     }
@@ -127,6 +135,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
                                                              const int *spill_count, PtsView in) {
     if (clk) record_pq = clk->record;
     const int count = *spill_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) count_stat(kp, kStatG2PSpilled, (unsigned long long)count);
     const double c4h = 4. * kp.h_inv;
     const BrickList bl = brick_list(gs_b, cur_b);
     for (int k = blockIdx.x * kP2GThreads + threadIdx.x; k < count; k += gridDim.x * kP2GThreads) {

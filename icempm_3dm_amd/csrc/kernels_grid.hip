@@ -107,9 +107,10 @@ __device__ __forceinline__ void update_bricks(const KParams &kp0, const GridStat
             const bool sphere = kp.ind_shape == ICEMPM_INDENTER_SPHERE;
             double cx = cx0, cy = cy0, cz = cz0;
             if constexpr (kEns) {
-                cx = kp.ind_c0[0] + t * kp.ind_vel[0];
-                cy = kp.ind_c0[1] + t * kp.ind_vel[1];
-                cz = kp.ind_c0[2] + t * kp.ind_vel[2];
+                // rounded as the one-scene path and the host law round it
+                cx = law_center(kp.ind_c0[0], t, kp.ind_vel[0]);
+                cy = law_center(kp.ind_c0[1], t, kp.ind_vel[1]);
+                cz = law_center(kp.ind_c0[2], t, kp.ind_vel[2]);
             }
             double *bins = bins0 + kp.bins_off;
             const int bz = b % kp.nbz;

@@ -83,6 +83,8 @@ __device__ __forceinline__ void tile_origin(const KParams &kp, int brick, int or
     for (int d = 0; d < 3; d++) org[d] = bc[d] * 4 - 2;
 }
 
+// Clears the tile and the brick mask. The mask is cleared by thread 0, a
+// lane of wavefront 0 like every reader of the mask (p2g_mark_tile_bricks).
 __device__ __forceinline__ void p2g_zero_tile(P2GShared &sh, int t) {
     for (int k = t; k < 4 * kTileAlloc; k += kP2GThreads) (&sh.tile[0][0])[k] = 0.0;
     if (t == 0) sh.brickmask = 0u;
@@ -253,6 +255,7 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
         }
         if (nover_counted > 0) {
             if (over) sh.over[offo + __popcll(om & ((1ull << lane) - 1ull))] = t;
+            if (t == 0) count_stat(kp, kStatP2GOutOfTile, (unsigned long long)nover_counted);
             __syncthreads();
         }
     } else {
@@ -265,6 +268,7 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
             int to = 0;
             for (int w = 0; w < kP2GThreads / 64; w++) to += sh.wave_count[w];
             sh.nover = to;
+            count_stat(kp, kStatP2GOutOfTile, (unsigned long long)to);
         }
         __syncthreads();
     }
@@ -379,10 +383,16 @@ __device__ __forceinline__ void p2g_flush_nodes(P2GShared &sh, int t, const KPar
 // Lists the bricks of sh.brickmask (after p2g_flush_nodes and a barrier).
 // Only lanes 0-26, all in wavefront 0, read the mask, so its clear by lane 0
 // (p2g_zero_tile) later in the same wavefront's program needs no barrier in
-// between.
+// between. The fused kernel relies on this (no barrier after the marking):
+// the asserts below fix the lane layout it needs, and building with
+// ICEMPM_SEGMENT_END_BARRIER=1 puts the barrier back to compare against.
+constexpr int kBrickMaskLanes = 27;
+static_assert(kBrickMaskLanes <= 64, "the brick-mask readers must all lie in wavefront 0");
+static_assert(kBrickMaskLanes <= 32, "the brick mask is one 32-bit word");
+static_assert(kP2GThreads % 64 == 0, "whole wavefronts per workgroup");
 __device__ __forceinline__ void p2g_mark_tile_bricks(const P2GShared &sh, int t, const KParams &kp, const int bc[3],
                                                      const BrickList &bl) {
-    if (t < 27 && ((sh.brickmask >> t) & 1u)) {
+    if (t < kBrickMaskLanes && ((sh.brickmask >> t) & 1u)) {
         const int dbx = t / 9, dby = (t / 3) % 3, dbz = t % 3;
         const long long b = ((long long)(bc[0] - 1 + dbx) * kp.nby + (bc[1] - 1 + dby)) * kp.nbz + (bc[2] - 1 + dbz);
         mark_brick(bl, b);

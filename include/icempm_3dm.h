@@ -220,12 +220,18 @@ int icempm_set_sort_period(icempm_ctx *ctx, int period);
 /* Extension: counters of the step (synchronizes). out[ICEMPM_STAT_SPILLED]:
  * points the last G2P found outside their brick tile (they take the slow
  * global-memory path); out[ICEMPM_STAT_SORTS]: brick regroupings done since
- * allocation; out[ICEMPM_STAT_SEGMENTS]: brick segments of the last one.
- * Writes min(n_out, ICEMPM_N_STATS) values. */
+ * allocation; out[ICEMPM_STAT_SEGMENTS]: brick segments of the last one;
+ * out[ICEMPM_STAT_P2G_OUT_OF_TILE]: points whose stencil a P2G scatter (plain,
+ * fused, or the adjoint's P2G-form scatter) found outside its segment's tile
+ * and added straight to the grid in HBM, summed since allocation;
+ * out[ICEMPM_STAT_G2P_SPILLED_TOTAL]: ICEMPM_STAT_SPILLED summed over every
+ * G2P since allocation. Writes min(n_out, ICEMPM_N_STATS) values. */
 #define ICEMPM_STAT_SPILLED 0
 #define ICEMPM_STAT_SORTS 1
 #define ICEMPM_STAT_SEGMENTS 2
-#define ICEMPM_N_STATS 3
+#define ICEMPM_STAT_P2G_OUT_OF_TILE 3
+#define ICEMPM_STAT_G2P_SPILLED_TOTAL 4
+#define ICEMPM_N_STATS 5
 int icempm_read_stats(icempm_ctx *ctx, long long *out, int n_out);
 
 /* Keys of the device points in device order (kind 0: reference key,
