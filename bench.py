@@ -74,6 +74,31 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpus() -> dict:
+    """The host's processors: the CPU model, os.cpu_count() and the CPUs this
+    process may run on (its affinity mask, the share of the host it is given)."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        share = os.cpu_count() or 1
+    return {"cpu_model": model, "host_cpus": os.cpu_count(), "process_cpus": share}
+
+
+def default_cpu_threads() -> int:
+    """Every CPU of this process's share, or OMP_NUM_THREADS where that is set lower."""
+    share = host_cpus()["process_cpus"]
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(share, int(env)) if env.isdigit() and int(env) > 0 else share
+
+
 def cpu_baseline(scene, prms, hs, max_seconds: float, max_steps: int, threads: int) -> dict:
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as orc
@@ -96,14 +121,16 @@ def cpu_baseline(scene, prms, hs, max_seconds: float, max_steps: int, threads: i
         if time.perf_counter() - t0 >= max_seconds:
             break
     dt = time.perf_counter() - t0
+    hc = host_cpus()
     return {
         "value": soa.size * steps / dt,
         "unit": "particle-updates/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{steps} steps of the same {soa.size}-point workload after 1 warm-up step "
-                  f"(oracle/mpm_oracle.cpp, OpenMP, {threads} threads, {os.uname().machine}); "
-                  f"{dt:.1f} s",
+                  f"(oracle/mpm_oracle.cpp, OpenMP, {threads} threads = every CPU of this process's share: "
+                  f"{hc['process_cpus']} of the host's {hc['host_cpus']}, {hc['cpu_model']}); {dt:.1f} s",
+        **hc,
     }
 
 
@@ -143,9 +170,18 @@ def run_steps(scene, hs, args, dt: float, timed: bool):
     g.synchronize()
     elapsed = time.perf_counter() - t0
     timings = g.read_timings() if timed else {}
+    if timed:
+        # one brick re-sort timed on its own (the timed window may hold none)
+        timings["sort_in_run"] = timings.pop("sort")
+        g.reset_timings()
+        g.sort_by_brick()
+        timings["sort"] = g.read_timings()["sort"]
     g.enable_timing(False)
     err = g.device_error_code()
     stats = g.read_stats()
+    g.transfer_from_device()
+    crushed = int(np.count_nonzero(g.hssoa.utility()[:g.hssoa.size] & (1 << 16)))
+    stats["crushed_fraction"] = crushed / max(g.hssoa.size, 1)
     active_nodes = 0
     if timed:
         # nodes holding mass after the last step (for the per-node algorithmic bytes)
@@ -326,11 +362,14 @@ def main() -> int:
     ap.add_argument("--scene", default="block_16m")
     ap.add_argument("--dt", type=float, default=2e-6)
     ap.add_argument("--unstable-dt", type=float, default=1e-5, help="0: skip the second run")
+    ap.add_argument("--unstable-steps", type=int, default=500,
+                    help="timed steps of the dt=1e-5 run (long enough for the points to crush and spread)")
     ap.add_argument("--fuse", type=int, default=1)
     ap.add_argument("--sort-period", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-max-steps", type=int, default=40)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host-core baseline threads (0: every CPU of this process's share)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble", type=int, default=64, help="scenes of the ensemble run; 0: skip it")
     ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint timing")
@@ -345,6 +384,8 @@ def main() -> int:
                     help="run only this part and print its result (not the bench line)")
     args = ap.parse_args()
 
+    if args.cpu_threads <= 0:
+        args.cpu_threads = default_cpu_threads()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs the "
               "single-partition step on one device)", file=sys.stderr)
@@ -381,6 +422,13 @@ def main() -> int:
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * args.steps / elapsed
+    # the re-sort's share per step at the run's period, whether or not the
+    # timed window happened to hold one
+    period = stats["launch"]["sort_period"]
+    sorts_in_window = timings.pop("sort_in_run", (0.0, 0))[1]
+    sort_ms = timings["sort"][0] / max(timings["sort"][1], 1)
+    owed = (args.steps / period if period > 0 else 0.0) - sorts_in_window
+    value_amortized = n * args.steps / (elapsed + owed * sort_ms * 1e-3)
     node_bytes = {"p2g": B_P2G_NODE, "update": B_UPD_NODE, "g2p": B_G2P_NODE, "reset": B_RESET_NODE,
                   "g2p2g": B_G2P_NODE + B_P2G_NODE, "sort": 0}
     pt_bytes = {"p2g": B_P2G_PT, "update": 0, "g2p": B_G2P_PT, "reset": 0, "g2p2g": B_G2P_PT + B_P2G_PT,
@@ -403,6 +451,7 @@ def main() -> int:
         key = f"{d['kernel']}@{scene.name}@dt={args.dt:g}"
         if key in tr:
             traffic = tr[key]["hbm_bytes_per_launch"]
+    traffic_gbs = traffic / (d["avg_ms"] * 1e-3) / 1e9 if traffic else None
     step_bytes = (B_P2G_PT + B_G2P_PT) * n + (B_P2G_NODE + B_UPD_NODE + B_G2P_NODE + B_RESET_NODE) * active_nodes
     roofline = {
         "bound": "hbm",
@@ -412,6 +461,8 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(d["alg_GBs"] / HBM_PEAK_GBS, 4),
         "traffic": traffic,
+        "traffic_GBs": round(traffic_gbs, 1) if traffic_gbs else None,
+        "traffic_frac": round(traffic_gbs / HBM_PEAK_GBS, 4) if traffic_gbs else None,
         "alg_bytes_per_launch": d["alg_bytes"],
         "step_alg_GBs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -419,11 +470,14 @@ def main() -> int:
 
     unstable = None
     if args.unstable_dt > 0:
-        log(f"bench: dt={args.unstable_dt}: {args.warmup} warm-up + {args.steps} timed steps")
-        el2, _, err2, _, stats2 = run_steps(scene, hs, args, args.unstable_dt, False)
-        unstable = {"dt": args.unstable_dt, "value": n * args.steps / el2,
-                    "ms_per_step": el2 * 1e3 / args.steps, "device_error": err2,
-                    "brick_segments_at_end": stats2["segments"], "launch": stats2["launch"]}
+        sub = argparse.Namespace(**vars(args))
+        sub.steps = args.unstable_steps
+        log(f"bench: dt={args.unstable_dt}: {args.warmup} warm-up + {sub.steps} timed steps")
+        el2, _, err2, _, stats2 = run_steps(scene, hs, sub, args.unstable_dt, False)
+        unstable = {"dt": args.unstable_dt, "steps": sub.steps, "value": n * sub.steps / el2,
+                    "ms_per_step": el2 * 1e3 / sub.steps, "device_error": err2,
+                    "brick_segments_at_end": stats2["segments"], "crushed_fraction": stats2["crushed_fraction"],
+                    "launch": stats2["launch"]}
 
     ppc = None
     if args.ppc_sweep:
@@ -480,6 +534,9 @@ def main() -> int:
             "launch": stats["launch"],
             "device_error": err,
         },
+        "value_with_amortized_sort": value_amortized,
+        "sorts_in_timed_window": sorts_in_window,
+        "crushed_fraction": stats["crushed_fraction"],
         "roofline": roofline,
         "phases": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in phases.items()},

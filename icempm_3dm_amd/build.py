@@ -77,6 +77,29 @@ def build(extra_flags: list[str] | None = None, jobs: int | None = None) -> Path
     return LIB
 
 
+CPP_DRIVERS = {"model3d_cycles": ROOT / "tests" / "cpp" / "model3d_cycles.cpp"}
+CPP_BIN = ROOT / "tests" / "cpp" / "bin"
+
+
+def build_cpp_drivers() -> list[Path]:
+    """Host programs written against the C++ mirrors of the reference's
+    classes (include/*.hpp), linked to the library: tests run them as a
+    C++ caller of the reference would."""
+    CPP_BIN.mkdir(parents=True, exist_ok=True)
+    out = []
+    for name, src in CPP_DRIVERS.items():
+        exe = CPP_BIN / name
+        tmp = exe.with_suffix(".tmp")
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), f"-L{LIB_DIR}",
+               "-licempm_3dm", "-Wl,-rpath,$ORIGIN/../../../icempm_3dm_amd/lib", "-o", str(tmp)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+        os.replace(tmp, exe)
+        out.append(exe)
+    return out
+
+
 def build_oracle() -> Path:
     """Builds the CPU oracle library (test infrastructure) with its Makefile."""
     res = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], capture_output=True, text=True)

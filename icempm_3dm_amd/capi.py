@@ -101,6 +101,45 @@ class SimParams3D(C.Structure):
     def SetPhi(self, phi_degrees: float) -> None:
         lib().icempm_params_set_phi(C.byref(self), C.c_double(phi_degrees))
 
+    # SimParams3D::ParseFile (parameters_sim_3d.cpp:88-160): keys and their
+    # meaning as the reference reads them; returns the raw point file named.
+    _PARSE_DOUBLE = ("InitialTimeStep", "YoungsModulus", "DomainDimensionX", "ParticleViewSize",
+                     "SimulationEndTime", "PoissonsRatio", "Gravity", "Density", "IndDiameter", "IndVelocity",
+                     "IndDepth", "IceCompressiveStrength", "IceTensileStrength", "IceTensileStrength2",
+                     "IceShearStrength", "DP_threshold_p", "GrainVariability", "ExtraSpaceForIncomingPoints",
+                     "RebalanceThresholdFreeSpaceRemaining", "RebalanceThresholdDisabledPercentage",
+                     "PointsTransferBufferFraction", "animation_threshold_pos")
+    _PARSE_INT = ("SetupType", "tpb_P2G", "tpb_Upd", "tpb_G2P", "GridHaloSize", "nPartitions", "GridY", "GridZ")
+
+    def ParseFile(self, file_name: str) -> str:
+        import json
+        import os
+        self.Reset()
+        if not os.path.exists(file_name):
+            raise IcempmError(ERR_INVALID_ARG, "configuration file is not found")
+        with open(file_name) as f:
+            doc = json.load(f)
+        if not isinstance(doc, dict):
+            raise IcempmError(ERR_INVALID_ARG, "configuration file is not JSON")
+        if "InputRawPoints" not in doc:
+            raise IcempmError(ERR_INVALID_ARG, "config parameter missing")
+        for k in self._PARSE_INT:
+            if k in doc:
+                setattr(self, k, int(doc[k]))
+        if "GridX" in doc:
+            self.GridXTotal = int(doc["GridX"])
+        for k in self._PARSE_DOUBLE:
+            if k in doc:
+                setattr(self, k, float(doc[k]))
+        if "BlockCutout" in doc:
+            for i in range(3):
+                self.BlockCutout[i] = float(doc["BlockCutout"][i])
+        if "DP_phi" in doc:
+            self.SetPhi(float(doc["DP_phi"]))
+        self.ComputeCamClayParams()
+        self.ComputeHelperVariables()
+        return str(doc["InputRawPoints"])
+
     def IndenterArraySize(self) -> int:
         """Number of doubles of the indenter accumulators (parameters_sim_3d.h:105)."""
         return 3 * self.IndenterSubdivisions * self.GridZ

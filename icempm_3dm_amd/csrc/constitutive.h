@@ -44,6 +44,20 @@ __device__ __forceinline__ void kirchhoff_stress_j(const KParams &kp, const doub
     P[8] += pterm;
 }
 
+// The same stress from J, J^{-2/3} and dev(F F^T) computed before (the values
+// kirchhoff_stress_j forms from F, so the result is the same).
+__device__ __forceinline__ void kirchhoff_stress_pre(const KParams &kp, double Je, double rc, const double devb[6],
+                                                     double P[9]) {
+    const double s = kp.mu * rc;
+    const double pterm = kp.kappa * 0.5 * (Je * Je - 1.);
+    P[0] = s * devb[0] + pterm;
+    P[4] = s * devb[1] + pterm;
+    P[8] = s * devb[2] + pterm;
+    P[1] = P[3] = s * devb[3];
+    P[2] = P[6] = s * devb[4];
+    P[5] = P[7] = s * devb[5];
+}
+
 __device__ __forceinline__ void kirchhoff_stress(const KParams &kp, const double F[9], double P[9]) {
     const double Je = det3(F);
     kirchhoff_stress_j(kp, F, Je, rcbrt(Je * Je), P);
@@ -77,32 +91,41 @@ __device__ __forceinline__ double rsq_f64(double x) {
 // sqrt(x) for x >= 0 (0 maps to 0; negative or NaN x gives NaN).
 __device__ __forceinline__ double sqrt_f64(double x) { return x == 0.0 ? 0.0 : x * rsq_f64(x); }
 
-// One cyclic-Jacobi rotation of the symmetric A in the (p,q) plane, zeroing
-// apq with Rutishauser's update; ar_p = A(r,p), ar_q = A(r,q) for the third
-// index r. V accumulates the rotation in its columns p and q.
-// t = tan(phi) = sgn(d) 2apq / (|d| + sqrt(d^2 + 4apq^2)), d = aqq - app,
-// the root of t^2 + 2 theta t - 1 = 0 with |phi| <= pi/4.
-__device__ __forceinline__ void jacobi_rotate(double &app, double &aqq, double &apq, double &arp, double &arq,
-                                              double V[9], int p, int q) {
-    const double d = aqq - app;
-    const double e2 = apq + apq;
-    const double r2 = fma(d, d, e2 * e2);
-    const double t0 = (d < 0.0 ? -e2 : e2) * rcp_f64(fabs(d) + r2 * rsq_f64(r2));
-    const double t = apq == 0.0 ? 0.0 : t0;
-    const double c = rsq_f64(fma(t, t, 1.0));
-    const double s = t * c;
-    app = fma(-t, apq, app);
-    aqq = fma(t, apq, aqq);
-    apq = 0.0;
-    const double rp = arp, rq = arq;
-    arp = c * rp - s * rq;
-    arq = s * rp + c * rq;
+// One one-sided (Hestenes) Jacobi rotation of the column pair (p, q) of A:
+// with alpha = |a_p|^2, beta = |a_q|^2 and gamma = a_p . a_q recomputed from
+// the columns themselves, the plane rotation J with
+// t = tan(phi) = sgn(d) 2 gamma / (|d| + sqrt(d^2 + 4 gamma^2)), d = beta - alpha
+// (the root of t^2 + 2 theta t - 1 = 0 with |phi| <= pi/4), c = 1/sqrt(1 + t^2),
+// s = t c, makes the two columns of A J orthogonal. V accumulates J in its
+// columns p and q. Returns whether the pair was already orthogonal to
+// working precision: |cos| = |gamma| / sqrt(alpha beta) <= 6.3e-16 (about 6
+// ulps; recomputed from rounded columns, |cos| does not fall much below a
+// few ulps, so a tighter test would only run the sweep limit).
+__device__ __forceinline__ bool hestenes_rotate(double A[9], double V[9], int p, int q) {
+    double al = 0.0, be = 0.0, ga = 0.0;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        const double vp = V[3 * r + p], vq = V[3 * r + q];
-        V[3 * r + p] = c * vp - s * vq;
-        V[3 * r + q] = s * vp + c * vq;
+        al = fma(A[3 * r + p], A[3 * r + p], al);
+        be = fma(A[3 * r + q], A[3 * r + q], be);
+        ga = fma(A[3 * r + p], A[3 * r + q], ga);
     }
+    const double d = be - al;
+    const double e2 = ga + ga;
+    const double r2 = fma(d, d, e2 * e2);
+    const double t0 = (d < 0.0 ? -e2 : e2) * rcp_f64(fabs(d) + r2 * rsq_f64(r2));
+    const double t = ga == 0.0 ? 0.0 : t0;
+    const double c = rsq_f64(fma(t, t, 1.0));
+    const double sn = t * c;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const double ap = A[3 * r + p], aq = A[3 * r + q];
+        A[3 * r + p] = c * ap - sn * aq;
+        A[3 * r + q] = sn * ap + c * aq;
+        const double vp = V[3 * r + p], vq = V[3 * r + q];
+        V[3 * r + p] = c * vp - sn * vq;
+        V[3 * r + q] = sn * vp + c * vq;
+    }
+    return ga * ga <= 4e-31 * (al * be);
 }
 
 __device__ __forceinline__ void swap_negate_cols(double V[9], int a, int b) {
@@ -115,52 +138,50 @@ __device__ __forceinline__ void swap_negate_cols(double V[9], int a, int b) {
     }
 }
 
+__device__ __forceinline__ double col_norm2(const double A[9], int c) {
+    return A[c] * A[c] + A[3 + c] * A[3 + c] + A[6 + c] * A[6 + c];
+}
+
 // F = U diag(S) V^T with U, V rotations (columns U[3r+k], V[3r+k]),
 // S[0] >= S[1] >= |S[2]| and S[2] < 0 iff det F < 0 — the convention of the
 // reference's svd3x3 (mpm_kernels.cuh:585-601), in FP64 instead of its FP32
 // McAdams routine (helper_math.cuh:23-1119).
-// V and S^2 from cyclic Jacobi on F^T F (converged to the last bit), sorted
-// descending; U by Gram-Schmidt on the columns of B = F V, its third column
-// the cross product of the first two, so S[2] = U_2 . B_2 carries the sign of
-// det F. Returns S^2 for the first two values as computed from B (S2[2] =
-// S[2]^2).
-__device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3], double S2[3], double V[9]) {
-    double a00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
-    double a11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
-    double a22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
-    double a01 = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
-    double a02 = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
-    double a12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+// One-sided Jacobi on F itself: cyclic sweeps of column rotations of
+// B = F V until every column pair is orthogonal to working precision, so the
+// singular values and vectors carry errors of rounding times cond(F), not
+// cond(F)^2 as an eigen-decomposition of F^T F would. Columns are then sorted
+// by norm, descending (det V kept +1), and U is taken by Gram-Schmidt on the
+// columns of B, its third column the cross product of the first two, so that
+// S[2] = U_2 . B_2 carries the sign of det F. S2 returns S^2.
+// svd3_inplace takes B = F and leaves F V in it.
+__device__ __forceinline__ void svd3_inplace(double B[9], double U[9], double S[3], double S2[3], double V[9]) {
 #pragma unroll
     for (int k = 0; k < 9; k++) V[k] = (k % 4 == 0) ? 1.0 : 0.0;
 #pragma unroll 1
     for (int sweep = 0; sweep < 12; sweep++) {
-        const double off = a01 * a01 + a02 * a02 + a12 * a12;
-        const double dia = a00 * a00 + a11 * a11 + a22 * a22;
-        if (!(off > 1e-34 * dia)) break; // off-diagonal below 1e-17 relative (or NaN)
-        jacobi_rotate(a00, a11, a01, a02, a12, V, 0, 1);
-        jacobi_rotate(a00, a22, a02, a01, a12, V, 0, 2);
-        jacobi_rotate(a11, a22, a12, a01, a02, V, 1, 2);
+        const bool c01 = hestenes_rotate(B, V, 0, 1);
+        const bool c02 = hestenes_rotate(B, V, 0, 2);
+        const bool c12 = hestenes_rotate(B, V, 1, 2);
+        if (c01 && c02 && c12) break; // also ends on NaN after 12 sweeps
     }
-    if (a00 < a11) {
-        const double t = a00; a00 = a11; a11 = t;
+    double n0 = col_norm2(B, 0), n1 = col_norm2(B, 1), n2 = col_norm2(B, 2);
+    if (n0 < n1) {
+        const double t = n0; n0 = n1; n1 = t;
         swap_negate_cols(V, 0, 1);
+        swap_negate_cols(B, 0, 1);
     }
-    if (a00 < a22) {
-        const double t = a00; a00 = a22; a22 = t;
+    if (n0 < n2) {
+        const double t = n0; n0 = n2; n2 = t;
         swap_negate_cols(V, 0, 2);
+        swap_negate_cols(B, 0, 2);
     }
-    if (a11 < a22) {
-        const double t = a11; a11 = a22; a22 = t;
+    if (n1 < n2) {
+        const double t = n1; n1 = n2; n2 = t;
         swap_negate_cols(V, 1, 2);
+        swap_negate_cols(B, 1, 2);
     }
-    double B[9];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) B[3 * r + c] = F[3 * r] * V[c] + F[3 * r + 1] * V[3 + c] + F[3 * r + 2] * V[6 + c];
     // first column
-    const double bb0 = B[0] * B[0] + B[3] * B[3] + B[6] * B[6];
+    const double bb0 = n0;
     double u0[3] = {1.0, 0.0, 0.0};
     S[0] = 0.0;
     if (bb0 > 0.0) {
@@ -203,6 +224,13 @@ __device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3]
     S2[2] = S[2] * S[2];
 }
 
+__device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3], double S2[3], double V[9]) {
+    double B[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) B[k] = F[k];
+    svd3_inplace(B, U, S, S2, V);
+}
+
 // ---------------------------------------------------------------- plasticity
 // Per-particle state of the return mapping (Point3D, point_3d.h:14-27).
 // Damage D of a point: the IEEE single in bits 32-63 of its utility word,
@@ -222,8 +250,18 @@ struct PState {
     double Fe[9];
     double Jp_inv;
     double p_tr, q_tr, Je_tr;
-    double rc_Jsq; // J^{-2/3} (rcbrt(Je_tr^2)), from compute_pq
+    double rc_Jsq;  // J^{-2/3} (rcbrt(Je_tr^2)), from compute_pq
+    double devb[6]; // dev(F F^T) of the trial F (xx, yy, zz, xy, xz, yz), from compute_pq
     long long util;
+};
+
+// What the Kirchhoff stress of an F needs besides the material constants:
+// J = det F, J^{-2/3} and dev(F F^T). G2P computes all three for the trial F
+// (compute_pq); when the return mapping leaves F as it is, the P2G record of
+// the next substep takes them from there instead of computing them again.
+struct StressPre {
+    bool valid;
+    double Je, rc, devb[6];
 };
 
 // ComputePQ (mpm_kernels.cuh:477-484)
@@ -243,6 +281,12 @@ __device__ __forceinline__ void compute_pq(const KParams &kp, PState &p) {
     for (int k = 0; k < 9; k++) sq += b[k] * b[k];
     p.rc_Jsq = rcbrt(Jsq);
     p.q_tr = kCoeff1 * kp.mu * p.rc_Jsq * sqrt(sq);
+    p.devb[0] = b[0];
+    p.devb[1] = b[4];
+    p.devb[2] = b[8];
+    p.devb[3] = b[1];
+    p.devb[4] = b[2];
+    p.devb[5] = b[5];
 }
 
 // GetParametersForGrain + CheckIfPointIsInsideFailureSurface (mpm_kernels.cuh:434-474)
@@ -293,24 +337,26 @@ __device__ __forceinline__ void usvt(const double U[9], const double s[3], const
 // f = 1 - dmg_soft * D, and D grows by the equivalent plastic shear strain
 // over the softening strain, stored as soon as the branch has fixed the new
 // shear stress. Without it the code is the reference's return mapping alone.
+// Returns whether F changed (false: inside the yield surface).
 template <bool kDamage = false>
-__device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
-    double U[9], V[9], sig[3], sig2[3];
-    svd3(p.Fe, U, sig, sig2, V);
+__device__ __forceinline__ bool return_map(const KParams &kp, PState &p) {
     constexpr bool damage = kDamage;
     const double *rmc = kp.rmc;
     const double pt0 = rmc[kRmcThresholdP];
     double pt = pt0;
     if constexpr (damage) pt *= 1.0 - rmc[kRmcDmgSoft] * damage_of(p.util);
-    double snew[3];
-    bool project = true;
-    if (p.p_tr < -pt || p.Jp_inv < 1) {
+    // The branch and the new shear stress depend on p, q and Jp_inv alone, so
+    // they are settled before the SVD, which then runs only for a point whose
+    // F changes and may consume F in place.
+    const bool tension = p.p_tr < -pt || p.Jp_inv < 1;
+    double q_n_1 = 0.0, s_tension = 0.0;
+    if (tension) {
         if (!damage || pt == pt0) {
-            snew[0] = snew[1] = snew[2] = rmc[kRmcCbrtJeNew];
+            s_tension = rmc[kRmcCbrtJeNew];
             p.Jp_inv *= rmc[kRmcJeNew] / p.Je_tr;
         } else {
             const double je_new = sqrt(-2. * (-pt) / kp.kappa + 1.);
-            snew[0] = snew[1] = snew[2] = cbrt(je_new);
+            s_tension = cbrt(je_new);
             p.Jp_inv *= je_new / p.Je_tr;
         }
         // the deviatoric stress drops from q_tr to 0
@@ -319,7 +365,6 @@ __device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
                                  fmin(1.0, damage_of(p.util) + p.q_tr / (3.0 * kp.mu) * rmc[kRmcDmgRate]));
     } else {
         const double pmax = kp.ice_comp;
-        double q_n_1;
         if (p.p_tr > pmax) {
             q_n_1 = 0;
         } else {
@@ -329,29 +374,35 @@ __device__ __forceinline__ void return_map(const KParams &kp, PState &p) {
                 2 * sqrt((pmax - p.p_tr) * (p.p_tr - pmin)) * kp.ice_shear / (pmax - pmin);
             q_n_1 = fmin(q_from_failure_surface, q_from_dp);
         }
-        project = p.q_tr >= q_n_1;
-        if (project) {
-            // the deviatoric stress drops from q_tr to q_n_1
-            if constexpr (damage)
-                p.util = with_damage(
-                    p.util, fmin(1.0, damage_of(p.util) + (p.q_tr - q_n_1) / (3.0 * kp.mu) * rmc[kRmcDmgRate]));
-            const double Je_tr_sq = p.Je_tr * p.Je_tr;
-            const double s = kp.mu * p.rc_Jsq;
-            const double mean = (sig2[0] + sig2[1] + sig2[2]) / 3;
-            double nv[3] = {s * (sig2[0] - mean), s * (sig2[1] - mean), s * (sig2[2] - mean)};
-            // Eigen normalized(): unchanged when the norm is zero
-            const double n2 = nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2];
-            if (n2 > 0) {
-                const double rn = rsq_f64(n2);
-#pragma unroll
-                for (int k = 0; k < 3; k++) nv[k] *= rn;
-            }
-            const double coef = q_n_1 * kCoeff1Inv * (Je_tr_sq * p.rc_Jsq * p.rc_Jsq) / kp.mu;
-#pragma unroll
-            for (int k = 0; k < 3; k++) snew[k] = sqrt_f64(coef * nv[k] + mean);
-        }
+        if (!(p.q_tr >= q_n_1)) return false; // inside the yield surface: F stays as it is
+        // the deviatoric stress drops from q_tr to q_n_1
+        if constexpr (damage)
+            p.util = with_damage(
+                p.util, fmin(1.0, damage_of(p.util) + (p.q_tr - q_n_1) / (3.0 * kp.mu) * rmc[kRmcDmgRate]));
     }
-    if (project) usvt(U, snew, V, p.Fe);
+    double U[9], V[9], sig[3], sig2[3];
+    svd3_inplace(p.Fe, U, sig, sig2, V); // p.Fe now holds F V, overwritten below
+    double snew[3];
+    if (tension) {
+        snew[0] = snew[1] = snew[2] = s_tension;
+    } else {
+        const double Je_tr_sq = p.Je_tr * p.Je_tr;
+        const double s = kp.mu * p.rc_Jsq;
+        const double mean = (sig2[0] + sig2[1] + sig2[2]) / 3;
+        double nv[3] = {s * (sig2[0] - mean), s * (sig2[1] - mean), s * (sig2[2] - mean)};
+        // Eigen normalized(): unchanged when the norm is zero
+        const double n2 = nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2];
+        if (n2 > 0) {
+            const double rn = rsq_f64(n2);
+#pragma unroll
+            for (int k = 0; k < 3; k++) nv[k] *= rn;
+        }
+        const double coef = q_n_1 * kCoeff1Inv * (Je_tr_sq * p.rc_Jsq * p.rc_Jsq) / kp.mu;
+#pragma unroll
+        for (int k = 0; k < 3; k++) snew[k] = sqrt_f64(coef * nv[k] + mean);
+    }
+    usvt(U, snew, V, p.Fe);
+    return true;
 }
 
 } // namespace icempm

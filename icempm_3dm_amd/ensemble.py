@@ -172,6 +172,33 @@ class BatchedEnsemble:
         return np.stack([self.scene_bins(raw, i).reshape(-1, 3).sum(axis=0) / float(self.params[i].UpdateEveryNthStep)
                          for i in range(self.n_scenes)])
 
+    def stored_grid_dims(self) -> tuple[int, int, int]:
+        """Nodes per axis of the stored grid that holds every scene's slot:
+        slots of ceil((largest scene grid + 4) / 4) bricks per axis, filled z
+        first, then y, then x, at most 1024 nodes per axis (the rule of
+        icempm_allocate_ensemble)."""
+        maxg = [max(getattr(p, a) for p in self.params) for a in ("GridXTotal", "GridY", "GridZ")]
+        slot_b = [(g + 4 + 3) // 4 for g in maxg]
+        maxslots = [1024 // (b * 4) for b in slot_b]
+        n = self.n_scenes
+        nz = min(n, maxslots[2])
+        ny = min((n + nz - 1) // nz, maxslots[1])
+        nx = (n + ny * nz - 1) // (ny * nz)
+        return slot_b[0] * nx * 4, slot_b[1] * ny * 4, slot_b[2] * nz * 4
+
+    def download_grid(self) -> np.ndarray:
+        """The stored grid as (4, X, Y, Z): mass and momentum (or velocity
+        after the grid update) of every node of every slot."""
+        dims = self.stored_grid_dims()
+        out = np.zeros(4 * dims[0] * dims[1] * dims[2])
+        check(lib().icempm_download_grid(self.ctx, dptr(out)))
+        return out.reshape(4, *dims)
+
+    def scene_grid(self, grid: np.ndarray, i: int) -> np.ndarray:
+        """Scene i's own nodes (4, GridXTotal, GridY, GridZ) of a stored grid."""
+        o, p = self.origin[i], self.params[i]
+        return grid[:, o[0]:o[0] + p.GridXTotal, o[1]:o[1] + p.GridY, o[2]:o[2] + p.GridZ]
+
     def read_timings(self) -> dict:
         ms = np.zeros(len(capi.PHASES))
         cnt = np.zeros(len(capi.PHASES), dtype=np.int64)

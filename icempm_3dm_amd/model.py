@@ -3,9 +3,11 @@
 ``Step`` runs one cycle of ``UpdateEveryNthStep`` substeps on the device
 (model_3d.cpp:35-51, without a host synchronization per substep), then
 copies the points and indenter accumulators back and evaluates the cycle
-force (model_3d.cpp:53-60, gpu_implementation6.cpp:417-510). The
-reference's logging, snapshot mutexes and multi-partition rebalance are not
-part of the step and are left out.
+force (model_3d.cpp:53-60, gpu_implementation6.cpp:417-510), and rebalances
+(model_3d.cpp:76-108) when the free space or the share of disabled points
+crosses its threshold: RemoveDisabledAndSort on the device, then the host
+buffer refreshed from it. The reference's logging, timing log and snapshot
+mutexes are not part of the step and are left out.
 """
 from __future__ import annotations
 
@@ -49,5 +51,8 @@ class Model3D:
         disabled = hs.count_disabled() / max(hs.capacity, 1)
         if free_space < p.RebalanceThresholdFreeSpaceRemaining or disabled > p.RebalanceThresholdDisabledPercentage:
             self.gpu.remove_disabled_and_sort()
+            # the reference sorts the host buffer and uploads it; here the
+            # device sorted, and the host buffer is refreshed from it
+            self.gpu.transfer_from_device()
             self.rebalances += 1
         return p.SimulationTime < p.SimulationEndTime

@@ -13,6 +13,7 @@
 #define ICEMPM_GPU_IMPLEMENTATION6_HPP
 
 #include <array>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -70,7 +71,8 @@ public:
         return simulation_time;
     }
     // gpu_implementation6.cpp:417-496: points, accumulators, error code,
-    // cycle force and the 1/(N h^2) scaling of the sensor bins
+    // cycle force, the 1/(N h^2) scaling of the sensor bins and the check
+    // that the enabled points still number prms.nPtsTotal
     void transfer_from_device(double *host_buffer, unsigned capacity, unsigned &size) {
         int n = 0;
         std::vector<double> raw(indenter_sensor_total.size());
@@ -80,6 +82,14 @@ public:
         evaluate_indenter_total_force();
         const double s = (double)prms_.UpdateEveryNthStep * prms_.cellsize * prms_.cellsize;
         for (double &v : indenter_sensor_total) v /= s;
+        long long count = 0;
+        for (unsigned i = 0; i < size; i++) {
+            long long u;
+            std::memcpy(&u, host_buffer + ICEMPM_IDX_UTILITY * (size_t)capacity + i, sizeof(u));
+            if (!(u & ICEMPM_STATUS_DISABLED)) count++;
+        }
+        if (count != prms_.nPtsTotal)
+            throw std::runtime_error("transfer_from_device(): active point count mismatch");
     }
     // gpu_implementation6.cpp:499-510
     void evaluate_indenter_total_force() {
@@ -101,6 +111,9 @@ public:
         throw_if(icempm_remove_disabled_and_sort(ctx_, &n));
         return n;
     }
+    // extensions: fusion of G2P with the next P2G, brick re-sort period
+    void set_fusion(bool on) { throw_if(icempm_set_fusion(ctx_, on ? 1 : 0)); }
+    void set_sort_period(int period) { throw_if(icempm_set_sort_period(ctx_, period)); }
 
     std::array<double, 3> indenter_force{{0.0, 0.0, 0.0}};
     std::vector<double> indenter_sensor_total;

@@ -1,0 +1,257 @@
+"""The five configurations at their full sizes on the device.
+
+Sizes where the oracle cannot follow (16M points) are checked through
+properties that hold at any size; the rest against the oracle itself:
+
+  16M block     8, 27 and 64 points per cell (16,000,000 / 15,998,796 /
+                16,000,000 points on 512x256x256), fused step with a re-sort
+                inside the run, lifted off the sticky band with the sphere
+                pressed into its top: no device error, points back in upload
+                order (grain bits, bounded displacement), grid mass equal to
+                the point mass, and the total momentum changed by the gravity
+                impulse minus dt x the indenter's recorded force alone.
+  64 scenes     the sweep ensemble (8.6M points): every scene's grid mass in
+                its own slot equals its point mass, no device error, and three
+                full-size scenes (smallest, middle, largest) against their own
+                oracle runs.
+  2M + damage   the first 50 substeps against the oracle with the damage
+                model (multi-threaded oracle runs; the bar is 10x the spread
+                between two summation orders and FMA contraction), then the
+                full 2000 substeps: D in [0, 1] and never decreasing.
+  100k block    as configured: dt = 1e-5 s, 200 substeps, indenter force every
+                10. At this dt the elastic CFL number is 2.33: the block
+                crushes and spreads and the force curve is chaotic, so the
+                device is held to the oracle's own sensitivity (the deviation
+                between the oracle with and without FMA contraction) or to
+                1% of the peak force, whichever is larger; mass exact and the
+                utility bits reported.
+"""
+from __future__ import annotations
+
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from icempm_3dm_amd import capi, scenes
+from icempm_3dm_amd.ensemble import BatchedEnsemble, sweep_scenes
+from parity_util import FIELDS, normwise
+from test_gpu_parity import _download, _gpu_with, _oracle_spread
+
+pytestmark = pytest.mark.gpu
+
+LINES = dict(FIELDS, P=[capi.IDX_P], Q=[capi.IDX_Q])
+
+
+# ------------------------------------------------------------------ 16M block
+@pytest.mark.parametrize("name", ["block_16m", "block_16m_ppc27", "block_16m_ppc64"])
+def test_block_16m_fused_run_conserves(require_gpu, name):
+    sc = replace(scenes.SCENES[name], dt=2e-6, indenter_speed=5.0)
+    p = scenes.make_params(sc)
+    hs = scenes.make_points(sc)
+    n = hs.size
+    h = p.cellsize
+    L = hs.lines()
+    L[capi.POSX + 1, :n] += 8 * h  # clear of the sticky band and every wall
+    shape, c, vel = sc.indenter()
+    law = (shape, (c[0], c[1] + 8 * h - 0.5 * h, c[2]), vel)
+    m = p.ParticleMass
+    util0 = hs.utility()[:n].copy()
+    pos0 = L[capi.POSX:capi.POSX + 3, :n].copy()
+    g = _gpu_with(p, hs, indenter=law)
+    g.set_fusion(True)
+    g.set_sort_period(10)
+    g.reset_indenter_force_accumulator()
+    nsub = 25
+    g.step(nsub, 0.0, 0)
+    stats = g.read_stats()
+    assert stats["sorts"] >= 3, stats  # the upload's sort and two inside the run
+    grid = g.download_grid()
+    mass = grid[0].sum()
+    del grid
+    bins = g.read_indenter_bins()
+    back = _download(g)
+    assert g.device_error_code() == 0
+    g.close()
+    Lb = back.lines()
+    assert np.array_equal(back.utility()[:n], util0), "points must come back in upload order"
+    disp = np.abs(Lb[capi.POSX:capi.POSX + 3, :n] - pos0).max()
+    assert disp < 0.5 * h, disp
+    assert mass == pytest.approx(n * m, rel=1e-12)
+    mom = m * Lb[capi.VELX:capi.VELX + 3, :n].sum(axis=1)
+    F = bins.reshape(-1, 3).sum(axis=0)
+    gravity = np.array([0.0, -p.Gravity * p.InitialTimeStep * nsub * m * n, 0.0])
+    expected = gravity - p.InitialTimeStep * F  # the block starts at rest
+    scale = m * np.abs(Lb[capi.VELX:capi.VELX + 3, :n]).sum() + p.InitialTimeStep * np.abs(F).sum()
+    # the indenter's impulse is far above the bar, so a force recorded with a
+    # wrong sign, mass or dt, or not at all, breaks the balance
+    assert p.InitialTimeStep * abs(F[1]) > 1e6 * 1e-12 * scale, "the indenter must touch the block"
+    assert np.all(np.abs(mom - expected) <= 1e-12 * scale), (mom, expected, scale)
+    print(name, n, "momentum residual / scale", np.abs(mom - expected).max() / scale, "sorts", stats["sorts"])
+
+
+# ------------------------------------------------------------------ 64-scene sweep
+def test_sweep_64_scenes(require_gpu):
+    sw = sweep_scenes(64, seed=2024, dt=2e-6)
+    clouds = [scenes.make_points(s) for s in sw]
+    starts = [c.copy() for c in clouds]
+    ens = BatchedEnsemble(sw, clouds=clouds)
+    assert ens.n_points > 8_000_000
+    ens.set_sort_period(20)
+    ens.reset_indenter_force_accumulator()
+    nsub = 6
+    ens.step(nsub)
+    raw = ens.transfer_from_device()
+    grid = ens.download_grid()
+    for i, p in enumerate(ens.params):
+        sg = ens.scene_grid(grid, i)
+        assert sg[0].sum() == pytest.approx(starts[i].size * p.ParticleMass, rel=1e-12), i
+        got = ens.scene_points(i)
+        assert np.array_equal(got[capi.IDX_UTILITY].view(np.int64), starts[i].utility()[:starts[i].size]), i
+    total = grid[0].sum()
+    assert total == pytest.approx(sum(s.size * p.ParticleMass for s, p in zip(starts, ens.params)), rel=1e-12)
+    del grid
+    counts = [s.size for s in starts]
+    order = np.argsort(counts)
+    for i in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        p = ens.params[i]
+        ind = capi.Indenter.make(*sw[i].indenter())
+        ref = starts[i].copy()
+        bins_ref = np.zeros(p.IndenterArraySize())
+        code, _ = orc.step(p, ind, ref.host_buffer, ref.capacity, ref.size, orc.grid_zeros(p), bins_ref, nsub, 0.0, 0,
+                           1)
+        assert code == 0
+        spread = _oracle_spread(p, ind, starts[i], nsub, ref)
+        got = ens.scene_points(i)
+        report = {}
+        for name, ls in LINES.items():
+            e = normwise(got[ls, :], ref.lines()[ls, :ref.size])
+            report[name] = (e, spread[name])
+            assert e <= max(1e-12, 10 * spread[name]), (i, name, report)
+        fd, fr = ens.scene_bins(raw, i).reshape(-1, 3).sum(0), bins_ref.reshape(-1, 3).sum(0)
+        assert normwise(fd, fr) <= 1e-9, (i, fd, fr)
+        print("scene", i, "points", ref.size, report)
+    ens.close()
+
+
+# ------------------------------------------------------------------ 2M block with damage
+DAMAGE = capi.Damage.make(**scenes.BLOCK_2M_DAMAGE)
+
+
+@pytest.fixture
+def oracle_damage():
+    orc.set_damage(DAMAGE)
+    yield
+    orc.set_damage(None)
+
+
+def test_block_2m_damage_against_oracle_then_full_run(require_gpu, oracle_damage):
+    sc = scenes.BLOCK_2M
+    p = scenes.make_params(sc)
+    law = sc.indenter()
+    ind = capi.Indenter.make(*law)
+    hs = scenes.make_points(sc)
+    n = hs.size
+    start = hs.copy()
+    g = _gpu_with(p, hs, indenter=law)
+    g.set_damage(DAMAGE)
+    g.set_sort_period(50)
+    nsub = 50
+    g.reset_indenter_force_accumulator()
+    g.step(nsub, 0.0, 0)
+    back = _download(g)
+    bins_dev = g.read_indenter_bins()
+    g.close()
+    runs = []
+    for kw in ({"nthreads": 16}, {"nthreads": 7}, {"nthreads": 16, "nofma": True}):
+        r = start.copy()
+        bins = np.zeros(p.IndenterArraySize())
+        code, _ = orc.step(p, ind, r.host_buffer, r.capacity, n, orc.grid_zeros(p), bins, nsub, 0.0, 0, **kw)
+        assert code == 0
+        runs.append((r, bins))
+    ref, bins_ref = runs[0]
+    report = {}
+    for name, ls in LINES.items():
+        spread = max(normwise(r.lines()[ls, :n], ref.lines()[ls, :n]) for r, _ in runs[1:])
+        e = normwise(back.lines()[ls, :n], ref.lines()[ls, :n])
+        report[name] = (e, spread)
+        assert e <= max(1e-12, 10 * spread), (name, report)
+    assert np.array_equal(back.utility()[:n] & 0xFFFFFFFF, ref.utility()[:n] & 0xFFFFFFFF)
+    assert normwise(bins_dev.reshape(-1, 3).sum(0), bins_ref.reshape(-1, 3).sum(0)) <= 1e-9
+    print("2M damage, 50 substeps", report)
+
+    # the configured run: 2000 substeps, force every 10, D checked every 500
+    hs2 = start.copy()
+    g = _gpu_with(p, hs2, indenter=law)
+    g.set_damage(DAMAGE)
+    g.set_sort_period(50)
+    D_prev = np.zeros(n)
+    t = 0.0
+    forces = []
+    for k in range(sc.steps // 10):
+        g.reset_indenter_force_accumulator()
+        t = g.step(10, t, 10 * k)
+        forces.append(g.read_indenter_bins().reshape(-1, 3).sum(0) / 10)
+        if (k + 1) % 50 == 0:
+            out = _download(g)
+            D = capi.damage_of(out.utility()[:n])
+            assert np.all((D >= 0) & (D <= 1)), k
+            assert np.all(D >= D_prev), (k, np.count_nonzero(D < D_prev))
+            D_prev = D
+    assert g.device_error_code() == 0
+    g.close()
+    forces = np.array(forces)
+    assert np.isfinite(forces).all() and np.abs(forces).max() > 0
+    print("2M damage, 2000 substeps: damaged points", int(np.count_nonzero(D_prev > 0)), "max D", D_prev.max(),
+          "peak force", np.linalg.norm(forces, axis=1).max())
+
+
+# ------------------------------------------------------------------ 100k block as configured
+def test_block_100k_as_configured(require_gpu):
+    sc = scenes.BLOCK_100K  # dt = 1e-5 s, 200 steps
+    assert sc.dt == 1e-5 and sc.steps == 200
+    p = scenes.make_params(sc)
+    law = sc.indenter()
+    ind = capi.Indenter.make(*law)
+    hs = scenes.make_points(sc)
+    n = hs.size
+    start = hs.copy()
+    g = _gpu_with(p, hs, indenter=law)
+    g.set_sort_period(10)
+    every = 10
+    fd, t = [], 0.0
+    for k in range(sc.steps // every):
+        g.reset_indenter_force_accumulator()
+        t = g.step(every, t, every * k)
+        fd.append(g.read_indenter_bins().reshape(-1, 3).sum(0) / every)
+    g.reset_grid()
+    g.p2g()
+    mass = g.download_grid()[0].sum()
+    back = _download(g)
+    assert g.device_error_code() == 0
+    g.close()
+    curves = []
+    finals = []
+    for kw in ({"nthreads": 8}, {"nthreads": 8, "nofma": True}):
+        r = start.copy()
+        fr, tr = [], 0.0
+        for k in range(sc.steps // every):
+            bins = np.zeros(p.IndenterArraySize())
+            code, tr = orc.step(p, ind, r.host_buffer, r.capacity, n, orc.grid_zeros(p), bins, every, tr, every * k,
+                                **kw)
+            assert code == 0
+            fr.append(bins.reshape(-1, 3).sum(0) / every)
+        curves.append(np.array(fr))
+        finals.append(r)
+    fd = np.array(fd)
+    peak = np.abs(curves[0]).max()
+    dev = np.abs(fd - curves[0]).max() / peak
+    self_dev = np.abs(curves[1] - curves[0]).max() / peak
+    crushed = [np.count_nonzero(x & capi.STATUS_CRUSHED) for x in (back.utility()[:n], finals[0].utility()[:n])]
+    flips = int(np.count_nonzero(back.utility()[:n] != finals[0].utility()[:n]))
+    print(f"100k at dt=1e-5, 200 substeps: force deviation {dev:.3e} of peak (oracle FMA on/off: {self_dev:.3e}); "
+          f"crushed device {crushed[0]} oracle {crushed[1]} of {n}; utility words differing {flips}")
+    assert mass == pytest.approx(n * p.ParticleMass, rel=1e-12)
+    assert dev <= max(0.01, 3 * self_dev), (dev, self_dev)
+    assert min(crushed) > 0.9 * n and abs(crushed[0] - crushed[1]) <= 0.01 * n, crushed

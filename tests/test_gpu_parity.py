@@ -182,13 +182,22 @@ def test_phases_match_oracle(require_gpu, liquid):
 def test_return_map_distorted_f(require_gpu):
     """Every point crushed, F far from a rotation: the vmax-clamped grid
     velocities of this state stretch F by up to ~50% in one step, both branches
-    of the return mapping run, and a few points invert (det F < 0). The
-    device's Jacobi-on-F^T F SVD then meets the oracle's one-sided Jacobi to
-    rounding times the conditioning of F, so besides the normwise bar, the
-    per-point error of Fe is held to 1e-13 * cond(F_trial)^2."""
+    of the return mapping run, and a few points invert (det F < 0). One point
+    in 40 starts nearly singular, F = R1 diag(1, 0.5, +-10^-u) R2^T with
+    cond(F) from 100 to 2000. The device's one-sided Jacobi SVD on F meets the
+    oracle's to rounding times the conditioning of F (a perturbation of F of
+    relative size e moves its polar factor by up to e cond(F)), so besides the
+    normwise bar, the per-point error of Fe is held to 1e-13 * cond(F_trial)."""
     p = scenes.make_params(SMALL)
     shape, center, vel = SMALL.indenter()
     hs = scenes.perturbed_points(SMALL, seed=5, amp_F=0.3, crushed_fraction=1.0)
+    rng = np.random.default_rng(55)
+    sel = np.arange(0, hs.size, 40)
+    for i in sel:
+        r1, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        r2, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        small = 10.0 ** -rng.uniform(2.0, 3.3) * (1 if i % 80 else -1)
+        hs.lines()[capi.FE00:capi.FE00 + 9, i] = (r1 @ np.diag([1.0, 0.5, small]) @ r2.T).reshape(9)
     ref = hs.copy()
     g = _gpu_with(p, hs, indenter=(shape, center, vel))
     g.reset_grid()
@@ -213,11 +222,12 @@ def test_return_map_distorted_f(require_gpu):
     F0 = hs.lines()[capi.FE00:capi.FE00 + 9, :n].T.reshape(-1, 3, 3)
     cond = np.maximum(np.linalg.cond(Fr), np.linalg.cond(F0))
     rel = np.linalg.norm(Fd - Fr, axis=(1, 2)) / np.linalg.norm(Fr, axis=(1, 2))
-    worst = int(np.argmax(rel / cond ** 2))
+    worst = int(np.argmax(rel / cond))
     inverted = int(np.count_nonzero(np.linalg.det(F0) < 0))
-    print("errors", errs, "max rel", rel.max(), "worst/cond^2", rel[worst] / cond[worst] ** 2, "cond", cond[worst],
-          "inverted inputs", inverted)
-    assert np.all(rel <= 1e-13 * cond ** 2), (rel[worst], cond[worst])
+    print("errors", errs, "max rel", rel.max(), "worst/cond", rel[worst] / cond[worst], "cond", cond[worst],
+          "max cond", cond.max(), "inverted inputs", inverted)
+    assert cond.max() > 1000, "the ill-conditioned points must be there"
+    assert np.all(rel <= 1e-13 * cond), (rel[worst], cond[worst])
     jp = back.lines()[capi.IDX_JP_INV, :n]
     assert np.count_nonzero(jp != hs.lines()[capi.IDX_JP_INV, :n]) > 0.1 * n, "tension branch not exercised"
     assert g.device_error_code() == 0

@@ -1,0 +1,356 @@
+"""Device paths that the other parity tests leave idle, against the oracle.
+
+  drift        points carried about two cells by a uniform velocity with no
+               re-sort: they leave their brick segment's tile, so the fused
+               and unfused P2G scatters send them to HBM directly and G2P
+               takes its spill path (both counted on the device and asserted
+               non-zero), in one scene and in a three-scene ensemble;
+  SetupType 2  the water-tank grid rules (no indenter contact, bottom band
+               stops downward motion only, drag), liquid points and grain ids
+               that read as negative shorts (utility low bits >= 32768);
+  raw cloud    the reference's own input: build/geometries_3d/3d_200k.h5 read
+               by the library's HDF5 reader, placed with the run configuration
+               input_data/p1.json (SimParams3D::ParseFile), regrouped by the
+               device's RemoveDisabledAndSort and stepped;
+  cycles       Model3D::Step's cycle semantics (model_3d.cpp:24-112): the
+               cycle force sum(bins) / N and the sensor totals bins / (N h^2)
+               (gpu_implementation6.cpp:438-459, 499-510) against the oracle's
+               accumulators, and a rebalance once disabled points pass
+               RebalanceThresholdDisabledPercentage, through the Python mirror
+               and through a C++ program built on include/icempm_model3d.hpp.
+Multi-substep comparisons use the bar of the other parity tests: per field
+max(1e-12, 10 x the oracle's own rounding spread), utility bits exact.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from icempm_3dm_amd import Model3D, capi, scenes, snapshot
+from icempm_3dm_amd.capi import check, dptr, lib
+from icempm_3dm_amd.ensemble import BatchedEnsemble
+from icempm_3dm_amd.hssoa import HostSideSOA
+from parity_util import FIELDS, normwise
+from test_gpu_parity import _download, _gpu_with, _oracle_spread
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLDEN = ROOT / "tests" / "golden"
+LINES = dict(FIELDS, P=[capi.IDX_P], Q=[capi.IDX_Q])
+
+
+def _assert_within_bar(got: np.ndarray, ref: HostSideSOA, spread: dict, n: int, tag=""):
+    report = {}
+    for name, ls in LINES.items():
+        e = normwise(got[ls, :n], ref.lines()[ls, :n])
+        report[name] = (e, spread[name])
+        assert e <= max(1e-12, 10 * spread[name]), (tag, name, report)
+    assert np.array_equal(got[capi.IDX_UTILITY, :n].view(np.int64), ref.utility()[:n]), tag
+    print(tag, report)
+    return report
+
+
+# ------------------------------------------------------------------ drift
+DRIFT_DT = 2e-6
+DRIFT_SUBSTEPS = 40
+
+
+def _drifting_block(seed: int, scene=None):
+    """A block lifted off the sticky band and clear of every wall, far from
+    its indenter, with no gravity, a uniform velocity that carries it about
+    two cells in 40 substeps and a small random state on top."""
+    sc = scene or replace(scenes.BLOCK_100K, cells=(12, 8, 12), dt=DRIFT_DT, gravity=0.0)
+    sc = replace(sc, dt=DRIFT_DT, gravity=0.0)
+    p = scenes.make_params(sc)
+    hs = scenes.perturbed_points(sc, seed=seed, amp_F=2e-4, amp_v=0.5, amp_B=20.0, crushed_fraction=0.05)
+    n = hs.size
+    h = p.cellsize
+    L = hs.lines()
+    L[capi.POSX + 1, :n] += 8 * h
+    v = 2.0 * h / (DRIFT_SUBSTEPS * DRIFT_DT)  # two cells over the run
+    L[capi.VELX:capi.VELX + 3, :n] += np.array([[v], [0.6 * v], [-0.8 * v]])
+    law = (capi.INDENTER_SPHERE, (-10.0, -10.0, -10.0), (0.0, 0.0, 0.0))
+    return sc, p, hs, law
+
+
+@pytest.mark.parametrize("fuse", [True, False])
+def test_drift_out_of_tile_paths_match_oracle(require_gpu, fuse):
+    sc, p, hs, law = _drifting_block(41)
+    ind = capi.Indenter.make(*law)
+    start, ref = hs.copy(), hs.copy()
+    g = _gpu_with(p, hs, indenter=law)
+    g.set_sort_period(0)
+    g.set_fusion(fuse)
+    s0 = g.read_stats()
+    t_dev = g.step(DRIFT_SUBSTEPS, 0.0, 0)
+    stats = g.read_stats()
+    code, t_ref = orc.step(p, ind, ref.host_buffer, ref.capacity, ref.size, orc.grid_zeros(p),
+                           np.zeros(p.IndenterArraySize()), DRIFT_SUBSTEPS, 0.0, 0, 1)
+    assert code == 0 and t_dev == t_ref
+    spread = _oracle_spread(p, ind, start, DRIFT_SUBSTEPS, ref)
+    back = _download(g)
+    n = ref.size
+    print("stats", stats)
+    assert stats["sorts"] == s0["sorts"], "no re-sort may run"
+    assert stats["g2p_spilled_total"] > n, "G2P spill path idle"
+    assert stats["p2g_out_of_tile"] > n, "P2G out-of-tile path idle"
+    _assert_within_bar(back.lines(), ref, spread, n, f"drift fuse={fuse}")
+    assert g.device_error_code() == 0
+    g.close()
+
+
+def test_drift_out_of_tile_paths_in_ensemble(require_gpu):
+    specs = [replace(scenes.BLOCK_100K, name="drift_a", cells=(12, 8, 12), seed=3),
+             replace(scenes.BLOCK_100K, name="drift_b", grid=(48, 40, 36), cells=(10, 9, 8), seed=4),
+             replace(scenes.BLOCK_100K, name="drift_c", grid=(72, 28, 40), cells=(14, 6, 10), seed=5)]
+    built = [_drifting_block(50 + i, s) for i, s in enumerate(specs)]
+    starts = [b[2].copy() for b in built]
+    ens = BatchedEnsemble([b[0] for b in built], clouds=[b[2] for b in built], params=[b[1] for b in built],
+                          indenters=[b[3] for b in built])
+    ens.set_sort_period(0)
+    ens.set_fusion(True)
+    ens.step(DRIFT_SUBSTEPS)
+    ens.transfer_from_device()
+    out = np.zeros(len(capi.STATS), dtype=np.int64)
+    check(lib().icempm_read_stats(ens.ctx, out.ctypes.data_as(C.POINTER(C.c_longlong)), len(capi.STATS)))
+    stats = dict(zip(capi.STATS, out.tolist()))
+    print("stats", stats)
+    assert stats["g2p_spilled_total"] > 0 and stats["p2g_out_of_tile"] > 0
+    for i, (sc, p, _, law) in enumerate(built):
+        ind = capi.Indenter.make(*law)
+        ref = starts[i].copy()
+        code, _ = orc.step(p, ind, ref.host_buffer, ref.capacity, ref.size, orc.grid_zeros(p),
+                           np.zeros(p.IndenterArraySize()), DRIFT_SUBSTEPS, 0.0, 0, 1)
+        assert code == 0
+        spread = _oracle_spread(p, ind, starts[i], DRIFT_SUBSTEPS, ref)
+        _assert_within_bar(ens.scene_points(i), ref, spread, ref.size, f"ensemble drift scene {i}")
+    ens.close()
+
+
+# ------------------------------------------------------------------ SetupType 2, liquid, negative grains
+def test_setup2_liquid_and_negative_grains_match_oracle(require_gpu):
+    sc = replace(scenes.BLOCK_100K, dt=2e-6)
+    p = scenes.make_params(sc)
+    p.SetupType = 2
+    p.DragGlobal = 0.03
+    shape, center, vel = sc.indenter()
+    # the sphere overlaps the block: SetupType 2 has no indenter contact
+    law = (shape, (center[0], center[1] - 4 * p.cellsize, center[2]), vel)
+    ind = capi.Indenter.make(*law)
+    hs = scenes.perturbed_points(sc, seed=77, amp_F=3e-3, amp_v=0.3, amp_B=5.0, crushed_fraction=0.1,
+                                 liquid_fraction=0.2)
+    n = hs.size
+    rng = np.random.default_rng(78)
+    util = hs.utility()
+    grains = rng.integers(0, 1 << 16, n)
+    util[:n] = (util[:n] & ~np.int64(0xFFFF)) | grains
+    assert np.count_nonzero(grains >= 32768) > n // 3
+    start, ref = hs.copy(), hs.copy()
+    g = _gpu_with(p, hs, indenter=law)
+    g.set_sort_period(3)
+    g.reset_indenter_force_accumulator()
+    nsub = 8
+    t_dev = g.step(nsub, 0.0, 0)
+    bins_ref = np.zeros(p.IndenterArraySize())
+    code, t_ref = orc.step(p, ind, ref.host_buffer, ref.capacity, n, orc.grid_zeros(p), bins_ref, nsub, 0.0, 0, 1)
+    assert code == 0 and t_dev == t_ref
+    spread = _oracle_spread(p, ind, start, nsub, ref)
+    back = _download(g)
+    _assert_within_bar(back.lines(), ref, spread, n, "setup2")
+    crushed = np.count_nonzero(back.utility()[:n] & capi.STATUS_CRUSHED)
+    assert crushed > 0.1 * n
+    assert np.all(g.read_indenter_bins() == 0) and np.all(bins_ref == 0)
+    g.close()
+
+
+# ------------------------------------------------------------------ the reference's own input
+def _p1_state():
+    p = capi.SimParams3D()
+    raw_name = p.ParseFile(str(GOLDEN / "p1.json"))
+    assert raw_name == "geometries_3d/3d_200k.h5"
+    p.nPartitions = 1  # one partition on one GPU
+    hs0, vol = snapshot.read_raw_points(str(GOLDEN / "3d_200k.h5"))
+    n = hs0.size
+    cap = int(n * (1 + p.ExtraSpaceForIncomingPoints))
+    hs = HostSideSOA(cap)
+    hs.lines()[:, :n] = hs0.lines()[:, :n]
+    hs.size = n
+    p.Volume = vol
+    check(lib().icempm_prepare_raw_points(C.byref(p), dptr(hs.host_buffer), hs.capacity, n))
+    return p, hs
+
+
+def test_reference_cloud_p1_end_to_end(require_gpu):
+    """3d_200k.h5 under p1.json: read, place, device sort (bit-exact against
+    the oracle's stable sort of the same placed points), then 20 substeps of
+    the reference's cylinder law against the oracle."""
+    p, hs = _p1_state()
+    n = hs.size
+    assert n == 202329 and p.nPtsTotal == n
+    ref = hs.copy()
+    assert orc.remove_disabled_and_sort(p, ref.host_buffer, ref.capacity, n) == n
+    g = _gpu_with(p, hs)  # update_constants resets the law to the reference cylinder
+    assert g.remove_disabled_and_sort() == n
+    back = _download(g)
+    assert np.array_equal(back.lines()[:, :n].view(np.int64), ref.lines()[:, :n].view(np.int64))
+    nsub = 20
+    g.set_sort_period(10)
+    g.reset_indenter_force_accumulator()
+    t_dev = g.step(nsub, 0.0, 0)
+    ind = orc.reference_indenter(p)
+    start = ref.copy()
+    bins_ref = np.zeros(p.IndenterArraySize())
+    code, t_ref = orc.step(p, ind, ref.host_buffer, ref.capacity, n, orc.grid_zeros(p), bins_ref, nsub, 0.0, 0, 8)
+    assert code == 0 and t_dev == t_ref
+    spread = _oracle_spread(p, ind, start, nsub, ref)
+    back = _download(g)
+    _assert_within_bar(back.lines(), ref, spread, n, "3d_200k p1")
+    fd, fr = g.read_indenter_bins().reshape(-1, 3).sum(0), bins_ref.reshape(-1, 3).sum(0)
+    assert np.abs(fr).sum() > 0, "the indenter must touch the block"
+    assert normwise(fd, fr) <= 1e-9
+    assert g.device_error_code() == 0
+    g.close()
+
+
+# ------------------------------------------------------------------ Model3D cycles
+CYCLE_N = 10
+
+
+def _cycle_scene(disabled_fraction: float):
+    sc = replace(scenes.BLOCK_100K, cells=(20, 12, 20), dt=2e-6, indenter_speed=3.0)
+    p = scenes.reference_cylinder_params(sc)
+    p.UpdateEveryNthStep = CYCLE_N
+    hs0 = scenes.perturbed_points(sc, seed=91, amp_F=5e-4, amp_v=0.02, amp_B=1.0, crushed_fraction=0.05)
+    n = hs0.size
+    hs = HostSideSOA(int(n * 1.25))
+    hs.lines()[:, :n] = hs0.lines()[:, :n]
+    hs.size = n
+    rng = np.random.default_rng(92)
+    dis = rng.random(n) < disabled_fraction
+    hs.utility()[:n][dis] |= capi.STATUS_DISABLED
+    p.nPtsTotal = int(n - dis.sum())
+    return p, hs
+
+
+def _oracle_cycles(p, hs: HostSideSOA, cycles: int, rebalance_after: int | None, nthreads: int = 1,
+                   nofma: bool = False):
+    """The oracle stepped cycle by cycle, with the raw accumulators of each
+    cycle and, after cycle `rebalance_after`, RemoveDisabledAndSort."""
+    ind = orc.reference_indenter(p)
+    ref = hs.copy()
+    t, step = 0.0, 0
+    raws = []
+    for c in range(cycles):
+        bins = np.zeros(p.IndenterArraySize())
+        code, t = orc.step(p, ind, ref.host_buffer, ref.capacity, ref.size, orc.grid_zeros(p), bins, CYCLE_N, t, step,
+                           nthreads, nofma=nofma)
+        assert code == 0
+        step += CYCLE_N
+        raws.append(bins)
+        if rebalance_after is not None and c == rebalance_after:
+            ref.size = orc.remove_disabled_and_sort(p, ref.host_buffer, ref.capacity, ref.size)
+    return ref, raws, t
+
+
+def _cycles_reference(p, hs: HostSideSOA, cycles: int, rebalance_after: int):
+    """The oracle's cycles and its rounding spread (another summation order,
+    no FMA contraction) per field."""
+    ref, raws, t = _oracle_cycles(p, hs, cycles, rebalance_after)
+    spread = {k: 0.0 for k in LINES}
+    for kw in ({"nthreads": 8}, {"nofma": True}):
+        other, _, _ = _oracle_cycles(p, hs, cycles, rebalance_after, **kw)
+        for name, ls in LINES.items():
+            spread[name] = max(spread[name], normwise(other.lines()[ls, :ref.size], ref.lines()[ls, :ref.size]))
+    return ref, raws, t, spread
+
+
+def _check_cycle_forces(p, forces, raws, tag):
+    """forces[c] = the cycle force Sum(bins) / N of cycle c (x, y, z)."""
+    peak = max(np.abs(r.reshape(-1, 3).sum(0)).max() for r in raws) / CYCLE_N
+    assert peak > 0, "the indenter must touch the block"
+    for c, raw in enumerate(raws):
+        want = raw.reshape(-1, 3).sum(0) / CYCLE_N            # gpu_implementation6.cpp:499-510
+        got = np.asarray(forces[c])
+        assert np.abs(got - want).max() <= 1e-9 * peak, (tag, c, got, want)
+
+
+def _check_sensor(p, sensor_last, raws, tag):
+    want = raws[-1] / (CYCLE_N * p.cellsize * p.cellsize)     # gpu_implementation6.cpp:459
+    assert np.abs(sensor_last - want).max() <= 1e-9 * np.abs(want).max(), tag
+
+
+def test_model3d_cycles_force_and_rebalance(require_gpu):
+    p, hs = _cycle_scene(0.08)
+    assert (hs.count_disabled() / hs.capacity) > p.RebalanceThresholdDisabledPercentage
+    start = hs.copy()
+    m = Model3D(p.copy())
+    m.gpu.hssoa = hs
+    m.gpu.allocate_arrays(m.prms, hs.capacity)
+    m.Prepare()
+    m.gpu.transfer_ponts_to_device()
+    m.gpu.set_sort_period(4)
+    cycles = 3
+    forces = []
+    for c in range(cycles):
+        m.Step()
+        forces.append(m.gpu.indenter_force.copy())
+        if c == 0:
+            assert m.rebalances == 1, "the disabled share is past the threshold: one rebalance"
+            assert hs.size == p.nPtsTotal and hs.count_disabled() == 0
+    assert m.rebalances == 1
+    assert m.prms.SimulationStep == cycles * CYCLE_N
+    ref, raws, t_ref, spread = _cycles_reference(p, start, cycles, rebalance_after=0)
+    assert m.prms.SimulationTime == t_ref
+    _check_cycle_forces(p, forces, raws, "python")
+    _check_sensor(p, m.gpu.indenter_sensor_total, raws, "python")
+    assert ref.size == hs.size
+    _assert_within_bar(hs.lines(), ref, spread, ref.size, "python model3d")
+    m.gpu.close()
+
+
+def test_cpp_model3d_cycles(require_gpu, tmp_path):
+    """The same cycles through tests/cpp/model3d_cycles (a C++ program on
+    include/icempm_model3d.hpp): force log, sensor totals, rebalance count
+    and final points against the oracle."""
+    exe = ROOT / "tests" / "cpp" / "bin" / "model3d_cycles"
+    assert exe.exists(), "build the C++ drivers first (python __graft_entry__.py)"
+    p, hs = _cycle_scene(0.08)
+    start = hs.copy()
+    law = orc.reference_indenter(p)
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        f.write(bytes(p))
+        f.write(bytes(law))
+        f.write(np.array([1, 4], dtype=np.int32).tobytes())
+        f.write(np.array([hs.capacity, hs.size], dtype=np.uint32).tobytes())
+        f.write(hs.host_buffer.tobytes())
+    cycles = 3
+    res = subprocess.run([str(exe), str(inp), str(outp), str(cycles)], capture_output=True, text=True, timeout=100)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0
+    raw = outp.read_bytes()
+    size, rebalances, ncyc = np.frombuffer(raw[:12], dtype=np.int32)
+    assert ncyc == cycles and rebalances == 1 and size == p.nPtsTotal
+    off = 12
+    log = np.frombuffer(raw[off:off + 32 * cycles], dtype=np.float64).reshape(cycles, 4)
+    off += 32 * cycles
+    nb = p.IndenterArraySize()
+    sensor = np.frombuffer(raw[off:off + 8 * nb], dtype=np.float64)
+    off += 8 * nb
+    pts = np.frombuffer(raw[off:], dtype=np.float64).reshape(capi.N_PTS_ARRAYS, hs.capacity)
+    ref, raws, t_ref, spread = _cycles_reference(p, start, cycles, rebalance_after=0)
+    assert log[-1, 0] == t_ref
+    for c in range(cycles):
+        want = raws[c].reshape(-1, 3).sum(0) / CYCLE_N
+        scale = max(np.abs(r.reshape(-1, 3).sum(0)).max() for r in raws) / CYCLE_N
+        assert np.abs(log[c, 2:4] - want[:2]).max() <= 1e-9 * scale, (c, log[c], want)
+        assert abs(log[c, 1] - np.linalg.norm(want)) <= 1e-9 * scale
+    _check_sensor(p, sensor, raws, "c++")
+    _assert_within_bar(pts, ref, spread, ref.size, "c++ model3d")

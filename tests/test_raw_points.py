@@ -25,7 +25,7 @@ from icempm_3dm_amd import capi, scenes, snapshot
 from icempm_3dm_amd.capi import check, dptr, lib
 from icempm_3dm_amd.hssoa import HostSideSOA
 
-RAW = Path("/root/reference/build/geometries_3d/3d_200k.h5")
+RAW = Path(__file__).resolve().parent / "golden" / "3d_200k.h5"
 need_raw = pytest.mark.skipif(not RAW.exists(), reason="reference input cloud not mounted here")
 
 
