@@ -1221,7 +1221,13 @@ int icempm_remove_disabled_and_sort(icempm_ctx *ctx, int *n_points_out) {
     const int n = ctx->n;
     if (n > 0) {
         PhaseTimer tm(ctx, ICEMPM_PHASE_SORT);
-        ICEMPM_HIP(launch_cell_keys(ctx->kp, ctx->view(ctx->cur_pts), n, 0, ctx->keys[0], ctx->vals[0], ctx->stream));
+        // Keys in upload order (the host buffer's order: slot maps device
+        // point -> upload index), so that the stable sort breaks ties between
+        // points of one cell by their place in the host buffer, as a stable
+        // sort of the host buffer does, whatever order the brick sorts left
+        // the device points in.
+        ICEMPM_HIP(launch_cell_keys(ctx->kp, ctx->view(ctx->cur_pts), n, 0, ctx->keys[0], ctx->vals[0], ctx->stream,
+                                    ctx->slot[ctx->cur_pts]));
         ICEMPM_HIP(radix_sort_pairs(ctx->keys[0], ctx->vals[0], ctx->keys[1], ctx->vals[1], n, ctx->scratch,
                                     ctx->stream));
         const int nxt = ctx->cur_pts ^ 1;

@@ -13,8 +13,11 @@ namespace icempm {
 // ---- sort (kernels_sort.hip) ----------------------------------------------
 // Key of every device point: kind 0 reference nearest-node key
 // (proxypoint3d.cpp:158-163), kind 1 brick key. Disabled points: 0xFFFFFFFF.
+// keys[o] and vals[o] (when vals is not null) receive the key of device
+// point i and i itself, o = order[i] when order is given (a permutation of
+// [0, n)), else o = i.
 hipError_t launch_cell_keys(const KParams &kp, PtsView pts, int n, int kind, uint32_t *keys,
-                            int32_t *vals, hipStream_t stream);
+                            int32_t *vals, hipStream_t stream, const int32_t *order = nullptr);
 // Scratch words needed by radix_sort_pairs for n items.
 size_t radix_sort_scratch_ints(int n);
 // Stable LSD radix sort (8-bit digits, 4 passes) of (keys, vals); the result

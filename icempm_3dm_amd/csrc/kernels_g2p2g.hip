@@ -109,8 +109,8 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
             Je = det3(s.Fe);
             rc = rcbrt(Je * Je);
         }
-        p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rc, s.Bp, s.Jp, org, err,
-                   pre.valid ? pre.devb : nullptr);
+        p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rc, s.Bp, s.Jp, org, err, pre.valid,
+                   pre.devb);
         // the record barrier also publishes the scatter's counts of points
         // outside the tile: one barrier fewer per chunk, and none for the
         // out-of-tile list when the counts are all zero

@@ -36,7 +36,10 @@ __device__ __forceinline__ int wave_inclusive_scan(int x) {
 }
 
 // ---------------------------------------------------------------- keys
-__global__ void k_cell_keys(KParams kp, PtsView pts, int n, int kind, uint32_t *keys, int32_t *vals) {
+// Key of device point i, written at position order[i] (i itself when order is
+// null) together with i in vals (when not null).
+__global__ void k_cell_keys(KParams kp, PtsView pts, int n, int kind, uint32_t *keys, int32_t *vals,
+                            const int32_t *order) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     long long util = double_to_bits(pts.at(ICEMPM_IDX_UTILITY, i));
@@ -66,8 +69,9 @@ __global__ void k_cell_keys(KParams kp, PtsView pts, int n, int kind, uint32_t *
         uint32_t local = (uint32_t)(((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3));
         key = brick * 64u + local;
     }
-    keys[i] = key;
-    if (vals) vals[i] = i;
+    const int o = order ? order[i] : i;
+    keys[o] = key;
+    if (vals) vals[o] = i;
 }
 
 __global__ void k_iota(int32_t *v, int n) {
@@ -265,9 +269,9 @@ inline int blocks_for(long long n, int threads) { return (int)((n + threads - 1)
 } // namespace
 
 hipError_t launch_cell_keys(const KParams &kp, PtsView pts, int n, int kind, uint32_t *keys, int32_t *vals,
-                            hipStream_t stream) {
+                            hipStream_t stream, const int32_t *order) {
     if (n <= 0) return hipSuccess;
-    k_cell_keys<<<blocks_for(n, 256), 256, 0, stream>>>(kp, pts, n, kind, keys, vals);
+    k_cell_keys<<<blocks_for(n, 256), 256, 0, stream>>>(kp, pts, n, kind, keys, vals, order);
     return hipGetLastError();
 }
 

@@ -123,6 +123,10 @@ def test_remove_disabled_and_sort(require_gpu):
     want = hs.copy()
     m = orc.remove_disabled_and_sort(p, want.host_buffer, want.capacity, want.size)
     g = _gpu_with(p, hs)
+    # a brick sort first: the device order is no longer the upload order, and
+    # points of one cell must still keep their host-buffer order
+    g.sort_by_brick()
+    assert not np.array_equal(g.device_order(), np.arange(hs.size))
     n_dev = g.remove_disabled_and_sort()
     assert n_dev == m
     g.hssoa.size = m

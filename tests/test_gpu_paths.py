@@ -99,8 +99,11 @@ def test_drift_out_of_tile_paths_match_oracle(require_gpu, fuse):
     n = ref.size
     print("stats", stats)
     assert stats["sorts"] == s0["sorts"], "no re-sort may run"
+    # the G2P spill list takes a point from the substep it leaves its tile to
+    # the end of the run; the P2G scatter sees it outside the tile only on
+    # the substep it crosses (the next G2P spills it), so fewer counts there
     assert stats["g2p_spilled_total"] > n, "G2P spill path idle"
-    assert stats["p2g_out_of_tile"] > n, "P2G out-of-tile path idle"
+    assert stats["p2g_out_of_tile"] > n // 10, "P2G out-of-tile path idle"
     _assert_within_bar(back.lines(), ref, spread, n, f"drift fuse={fuse}")
     assert g.device_error_code() == 0
     g.close()
