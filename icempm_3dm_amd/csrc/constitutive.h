@@ -44,20 +44,6 @@ __device__ __forceinline__ void kirchhoff_stress_j(const KParams &kp, const doub
     P[8] += pterm;
 }
 
-// The same stress from J, J^{-2/3} and dev(F F^T) computed before (the values
-// kirchhoff_stress_j forms from F, so the result is the same).
-__device__ __forceinline__ void kirchhoff_stress_pre(const KParams &kp, double Je, double rc, const double devb[6],
-                                                     double P[9]) {
-    const double s = kp.mu * rc;
-    const double pterm = kp.kappa * 0.5 * (Je * Je - 1.);
-    P[0] = s * devb[0] + pterm;
-    P[4] = s * devb[1] + pterm;
-    P[8] = s * devb[2] + pterm;
-    P[1] = P[3] = s * devb[3];
-    P[2] = P[6] = s * devb[4];
-    P[5] = P[7] = s * devb[5];
-}
-
 __device__ __forceinline__ void kirchhoff_stress(const KParams &kp, const double F[9], double P[9]) {
     const double Je = det3(F);
     kirchhoff_stress_j(kp, F, Je, rcbrt(Je * Je), P);
@@ -250,18 +236,8 @@ struct PState {
     double Fe[9];
     double Jp_inv;
     double p_tr, q_tr, Je_tr;
-    double rc_Jsq;  // J^{-2/3} (rcbrt(Je_tr^2)), from compute_pq
-    double devb[6]; // dev(F F^T) of the trial F (xx, yy, zz, xy, xz, yz), from compute_pq
+    double rc_Jsq; // J^{-2/3} (rcbrt(Je_tr^2)), from compute_pq
     long long util;
-};
-
-// What the Kirchhoff stress of an F needs besides the material constants:
-// J = det F, J^{-2/3} and dev(F F^T). G2P computes all three for the trial F
-// (compute_pq); when the return mapping leaves F as it is, the P2G record of
-// the next substep takes them from there instead of computing them again.
-struct StressPre {
-    bool valid;
-    double Je, rc, devb[6];
 };
 
 // ComputePQ (mpm_kernels.cuh:477-484)
@@ -281,12 +257,6 @@ __device__ __forceinline__ void compute_pq(const KParams &kp, PState &p) {
     for (int k = 0; k < 9; k++) sq += b[k] * b[k];
     p.rc_Jsq = rcbrt(Jsq);
     p.q_tr = kCoeff1 * kp.mu * p.rc_Jsq * sqrt(sq);
-    p.devb[0] = b[0];
-    p.devb[1] = b[4];
-    p.devb[2] = b[8];
-    p.devb[3] = b[1];
-    p.devb[4] = b[2];
-    p.devb[5] = b[5];
 }
 
 // GetParametersForGrain + CheckIfPointIsInsideFailureSurface (mpm_kernels.cuh:434-474)

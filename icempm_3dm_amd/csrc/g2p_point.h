@@ -197,13 +197,10 @@ __device__ __forceinline__ void gather_global(const KParams &kp, const double *g
 // here and false is returned; otherwise the trial F is stored and true is
 // returned for a crushed solid point, whose return mapping must follow
 // before its F is used. With kPreloaded, s.Fe and s.Jp already hold the
-// stored values. kDamage: the return mapping with the damage model. pre (may
-// be null) receives J, J^{-2/3} and dev(F F^T) of the new F when they are
-// those compute_pq formed (a solid point whose F the return mapping left as
-// it was), with pre->valid telling.
+// stored values. kDamage: the return mapping with the damage model.
 template <bool kInlineReturnMap, bool kPreloaded = false, bool kDamage = false>
 __device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i, PointState &s, int record_pq,
-                                           bool store_vc = true, StressPre *pre = nullptr) {
+                                           bool store_vc = true) {
     const double dt = kp.dt;
     const bool liquid = s.util & ICEMPM_STATUS_LIQUID;
 #pragma unroll
@@ -234,26 +231,15 @@ __device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i
     p.p_tr = 0.0;
     p.q_tr = 0.0;
     bool crushed = false;
-    bool kept = false; // a solid point whose trial F stays
     if (liquid) {
         p.Jp_inv *= det3(A);
     } else {
         compute_pq(kp, p);
         if (!(p.util & ICEMPM_STATUS_CRUSHED)) check_failure_surface(kp, p);
         crushed = (p.util & ICEMPM_STATUS_CRUSHED) != 0;
-        kept = !crushed;
         if (kInlineReturnMap && crushed) {
-            kept = !return_map<kDamage>(kp, p);
+            return_map<kDamage>(kp, p);
             crushed = false;
-        }
-    }
-    if (pre) {
-        pre->valid = kept;
-        if (kept) {
-            pre->Je = p.Je_tr;
-            pre->rc = p.rc_Jsq;
-#pragma unroll
-            for (int k = 0; k < 6; k++) pre->devb[k] = p.devb[k];
         }
     }
 #pragma unroll

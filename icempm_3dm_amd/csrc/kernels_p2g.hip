@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(KParams kp, PtsView pts, Se
         if (c.first()) p2g_zero_tile(sh, t);
         const double Je = (in.util & ICEMPM_STATUS_DISABLED) ? 1.0 : det3(in.Fe);
         p2g_record(sh, t, ks, c.c0 + t < c.end, in.util, in.pos, in.vel, in.Fe, Je, rcbrt(Je * Je), in.Bp, in.Jp,
-                   org, err, false, nullptr);
+                   org, err);
         const ChunkCursor nx = cursor_next(c, seg);
         if (nx.valid()) load_p2g_inputs(pts, nx.c0 + t, nx.c0 + t < nx.end, in);
         __syncthreads();

@@ -92,15 +92,12 @@ __device__ __forceinline__ void p2g_zero_tile(P2GShared &sh, int t) {
 
 // Builds the LDS record of point slot t from its state (the body of
 // partition_kernel_p2g up to the scatter, mpm_kernels.cuh:27-81); Je = det Fe
-// and rc = rcbrt(Je^2) are passed in for solid points, and dev(Fe Fe^T) too
-// when use_devb is set (devb is read only then: it is always a valid array,
-// never a null pointer, so that the caller's array stays in registers).
-// Points with a stencil outside the grid raise the error code and get no
-// record.
+// and rc = rcbrt(Je^2) are passed in for solid points. Points with a stencil
+// outside the grid raise the error code and get no record.
 __device__ __forceinline__ void p2g_record(P2GShared &sh, int t, const KParams &kp, bool valid, long long util,
                                            const double pos[3], const double vel[3], const double Fe[9],
                                            double Je, double rc, const double Bp[9], double Jp_inv,
-                                           const int org[3], int *err, bool use_devb, const double devb[6]) {
+                                           const int org[3], int *err) {
     int packed = -1;
     if (valid && !(util & ICEMPM_STATUS_DISABLED)) {
         const bool liquid = util & ICEMPM_STATUS_LIQUID;
@@ -115,7 +112,6 @@ __device__ __forceinline__ void p2g_record(P2GShared &sh, int t, const KParams &
             {
                 double PFt[9];
                 if (liquid) water_stress(kp, Jp_inv, PFt);
-                else if (use_devb) kirchhoff_stress_pre(kp, Je, rc, devb, PFt);
                 else kirchhoff_stress_j(kp, Fe, Je, rc, PFt);
 #pragma unroll
                 for (int k = 0; k < 9; k++) sub[k] = m * Bp[k] - kp.dt_vol_Dpinv * PFt[k];
