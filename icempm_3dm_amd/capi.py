@@ -258,6 +258,7 @@ _PROTOS = {
     "icempm_cell_keys": (C.c_int, [_ctxp, C.c_int, _u32p]),
     "icempm_device_order": (C.c_int, [_ctxp, _i32p]),
     "icempm_radix_sort_pairs": (C.c_int, [_ctxp, _u32p, C.c_int, _u32p, _i32p]),
+    "icempm_radix_sort_pairs_noalloc": (C.c_int, [_ctxp, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "icempm_download_grid": (C.c_int, [_ctxp, _dp]),
     "icempm_enable_timing": (C.c_int, [_ctxp, C.c_int]),
     "icempm_reset_timings": (C.c_int, [_ctxp]),

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -202,6 +203,36 @@ namespace {
 
 int set_device(icempm_ctx *ctx) {
     ICEMPM_HIP(hipSetDevice(ctx->device));
+    return ICEMPM_OK;
+}
+
+// ---------------------------------------------------------------- no-allocation sort
+// Device memory of icempm_radix_sort_pairs_noalloc: keys and values of the
+// ping-pong passes (ICEMPM_NOALLOC_MAX_PAIRS each), then the digit counts and
+// scan partials (radix_sort_scratch_ints, checked against kNoAllocCountWords
+// before use). One copy per GPU, held until the process ends. Host code never
+// passes `scratch` itself to a kernel (on the host it is a stand-in, not a
+// device address): it uses the per-device address from hipGetSymbolAddress.
+constexpr size_t kNoAllocPairs = ICEMPM_NOALLOC_MAX_PAIRS;
+constexpr size_t kNoAllocCountWords = kNoAllocPairs / 16 + 4096;
+constexpr size_t kNoAllocWords = 2 * kNoAllocPairs + kNoAllocCountWords;
+static __device__ __align__(256) int32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+int32_t *g_noalloc_ptr[kMaxDevices] = {};
+std::mutex g_noalloc_mutex;
+
+// Looks up the array's address on `device` (current) once; icempm_create
+// calls it for the device of every new context.
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(ICEMPM_ERR_INVALID_ARG, "device index beyond the table");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_ptr[device]) return ICEMPM_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(ICEMPM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[device] = static_cast<int32_t *>(ptr);
     return ICEMPM_OK;
 }
 
@@ -696,6 +727,7 @@ int icempm_create(int device, icempm_ctx **out) {
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(ICEMPM_ERR_NO_DEVICE, "no HIP device");
     if (device < 0 || device >= count) return fail(ICEMPM_ERR_INVALID_ARG, "device index out of range");
     ICEMPM_HIP(hipSetDevice(device));
+    ICEMPM_TRY(noalloc_setup(device));
     icempm_ctx *ctx = new icempm_ctx();
     ctx->device = device;
     hipError_t e = hipStreamCreate(&ctx->stream);
@@ -1446,6 +1478,32 @@ int icempm_radix_sort_pairs(icempm_ctx *ctx, const uint32_t *keys, int n, uint32
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     cleanup();
     if (e != hipSuccess) return fail(ICEMPM_ERR_HIP, std::string("radix_sort_pairs: ") + hipGetErrorString(e));
+    return ICEMPM_OK;
+}
+
+int icempm_radix_sort_pairs_noalloc(icempm_ctx *ctx, const uint32_t *keys_dev, int n, uint32_t *keys_sorted_dev,
+                                    int32_t *perm_dev) {
+    if (!ctx) return fail(ICEMPM_ERR_INVALID_ARG, "null context");
+    if (n < 0 || (n > 0 && (!keys_dev || !keys_sorted_dev || !perm_dev)))
+        return fail(ICEMPM_ERR_INVALID_ARG, "bad arguments");
+    if ((size_t)n > kNoAllocPairs)
+        return fail(ICEMPM_ERR_INVALID_ARG, "radix_sort_pairs_noalloc: n = " + std::to_string(n) +
+                                                " exceeds ICEMPM_NOALLOC_MAX_PAIRS = " + std::to_string(kNoAllocPairs));
+    if (radix_sort_scratch_ints(n) > kNoAllocCountWords)
+        return fail(ICEMPM_ERR_INVALID_ARG, "radix_sort_pairs_noalloc: digit counts exceed the fixed array");
+    ICEMPM_TRY(set_device(ctx));
+    if (n == 0) return ICEMPM_OK;
+    int32_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        base = g_noalloc_ptr[ctx->device];
+    }
+    if (!base) return fail(ICEMPM_ERR_NOT_READY, "radix_sort_pairs_noalloc: no fixed array on this device");
+    uint32_t *keys_tmp = reinterpret_cast<uint32_t *>(base);
+    int32_t *vals_tmp = base + kNoAllocPairs;
+    int32_t *counts = base + 2 * kNoAllocPairs;
+    ICEMPM_HIP(launch_copy_keys_iota(keys_dev, keys_sorted_dev, perm_dev, n, ctx->stream));
+    ICEMPM_HIP(radix_sort_pairs(keys_sorted_dev, perm_dev, keys_tmp, vals_tmp, n, counts, ctx->stream));
     return ICEMPM_OK;
 }
 

@@ -38,6 +38,8 @@ hipError_t launch_brick_segments(const uint32_t *sorted_keys, int n, int32_t *se
 // *count_out = number of keys != 0xFFFFFFFF in a sorted key array.
 hipError_t launch_count_valid(const uint32_t *sorted_keys, int n, int32_t *count_out, hipStream_t stream);
 hipError_t launch_iota(int32_t *v, int n, hipStream_t stream);
+// out[i] = in[i] (in == out allowed) and idx[i] = i.
+hipError_t launch_copy_keys_iota(const uint32_t *in, uint32_t *out, int32_t *idx, int n, hipStream_t stream);
 // Device-wide exclusive prefix sum of n int32 (in place allowed).
 hipError_t exclusive_scan(const int32_t *in, int32_t *out, int n, int32_t *scratch, hipStream_t stream);
 size_t exclusive_scan_scratch_ints(int n);

@@ -74,6 +74,14 @@ __global__ void k_cell_keys(KParams kp, PtsView pts, int n, int kind, uint32_t *
     if (vals) vals[o] = i;
 }
 
+__global__ void k_copy_keys_iota(const uint32_t *in, uint32_t *out, int32_t *idx, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = in[i];
+        idx[i] = i;
+    }
+}
+
 __global__ void k_iota(int32_t *v, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v[i] = i;
@@ -272,6 +280,12 @@ hipError_t launch_cell_keys(const KParams &kp, PtsView pts, int n, int kind, uin
                             hipStream_t stream, const int32_t *order) {
     if (n <= 0) return hipSuccess;
     k_cell_keys<<<blocks_for(n, 256), 256, 0, stream>>>(kp, pts, n, kind, keys, vals, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_keys_iota(const uint32_t *in, uint32_t *out, int32_t *idx, int n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_copy_keys_iota<<<blocks_for(n, 256), 256, 0, stream>>>(in, out, idx, n);
     return hipGetLastError();
 }
 

@@ -243,6 +243,22 @@ int icempm_device_order(icempm_ctx *ctx, int32_t *slot_out);
  * perm_out are host arrays; keys_sorted_out/perm_out receive the result. */
 int icempm_radix_sort_pairs(icempm_ctx *ctx, const uint32_t *keys, int n, uint32_t *keys_sorted_out,
                             int32_t *perm_out);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the sort of icempm_radix_sort_pairs on DEVICE arrays,
+ * allocating nothing. keys_dev holds n keys; keys_sorted_dev receives them
+ * sorted and perm_dev the stable order (keys_sorted_dev[i] =
+ * keys_dev[perm_dev[i]]); keys_sorted_dev may equal keys_dev. The sort's
+ * ping-pong arrays and digit counts live in one fixed device array of the
+ * library per GPU (about 35 MB, set up by icempm_create), so
+ * n <= ICEMPM_NOALLOC_MAX_PAIRS (larger n: ICEMPM_ERR_INVALID_ARG). The work
+ * is enqueued on the context's stream and the call returns without waiting
+ * (icempm_synchronize waits). Every context on one GPU shares that GPU's
+ * array: calls that overlap in time on one GPU (two streams, two host
+ * threads) race and must be serialised by the caller. */
+#define ICEMPM_NOALLOC_MAX_PAIRS (1 << 22)
+int icempm_radix_sort_pairs_noalloc(icempm_ctx *ctx, const uint32_t *keys_dev, int n, uint32_t *keys_sorted_dev,
+                                    int32_t *perm_dev);
 /* Copies the grid out in the reference's x-major node order
  * (x*GY*GZ + y*GZ + z), 4 lines of GX*GY*GZ doubles (m, vx|mvx, vy|mvy, vz|mvz). */
 int icempm_download_grid(icempm_ctx *ctx, double *out);
