@@ -354,6 +354,29 @@ def run_adjoint(args) -> dict:
             "dL_dE_ones": float(g[0]), "force_last": [float(x) for x in F[-1]]}
 
 
+def copy_calibration(nbytes: int) -> int:
+    """One device-to-device hipMemcpy of nbytes (HIP runtime of the library)."""
+    import ctypes as C
+
+    from icempm_3dm_amd import capi
+
+    capi.lib()  # the library's HIP runtime is the one found by soname below
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), nbytes) or hip.hipMalloc(C.byref(dst), nbytes):
+        print("copy calibration: hipMalloc failed", file=sys.stderr)
+        return 1
+    rc = hip.hipMemset(src, 1, nbytes) or hip.hipMemcpy(dst, src, nbytes, 3) or hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "rc": rc}))
+    return 1 if rc else 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,7 +405,12 @@ def main() -> int:
     ap.add_argument("--blocks-per-cu", type=int, default=8)
     ap.add_argument("--only", choices=("ensemble", "adjoint", "damage"), default=None,
                     help="run only this part and print its result (not the bench line)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES between two device buffers with hipMemcpy and exit (the copy of known "
+                         "size that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
 
     if args.cpu_threads <= 0:
         args.cpu_threads = default_cpu_threads()

@@ -12,6 +12,10 @@ struct PointState {
     long long util;
     double pos[3], vel[3], Fe[9], Bp[9];
     double Jp;
+    // J = det Fe and J^{-2/3} of the stored Fe, when g2p_finish formed them
+    // (a solid point whose trial F the return mapping left as it was)
+    double Je, rcJ;
+    bool have_j;
 };
 
 // Loads what G2P needs before the gather: utility and position. Fe and
@@ -65,6 +69,31 @@ __device__ __forceinline__ void vel_tile_store(double (*sv)[kTileAlloc], const V
     }
 }
 static_assert(kTileNodes == 2 * kP2GThreads, "two tile nodes per thread");
+
+// Start of a segment that follows the next brick down in z: its tile is the
+// previous one moved up 4 nodes. Thread t takes element q = t & 3 of row
+// (tx, ty) = (t >> 5, (t >> 2) & 7): the previous tile's element q + 4 moves
+// down to q, and element q + 4 is read from the grid (node (tx, ty, q + 4) of
+// the new tile). Call after the previous segment's gathers are finished.
+__device__ __forceinline__ void vel_tile_slide(double (*sv)[kTileAlloc], int t, const KParams &kp,
+                                               const int org[3], const double *grid) {
+    const int tx = t >> 5, ty = (t >> 2) & 7, q = t & 3;
+    const int x = org[0] + tx, y = org[1] + ty, z = org[2] + 4 + q;
+    double v[3] = {0.0, 0.0, 0.0};
+    // nodes outside the allocated bricks are never weighted (their points raised an error)
+    if ((unsigned)x < (unsigned)(kp.nbx * 4) && (unsigned)y < (unsigned)(kp.nby * 4) &&
+        (unsigned)z < (unsigned)(kp.nbz * 4)) {
+        const long long idx = node_index(kp, x, y, z);
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = grid[(1 + c) * kp.grid_pitch + idx];
+    }
+    const int lo = tile_index(tx, ty, q);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        sv[c][lo] = sv[c][lo + 4];
+        sv[c][lo + 4] = v[c];
+    }
+}
 
 // One wavefront loads the whole tile: lane l takes the x-planes' nodes
 // (x, l >> 3, l & 7), all 24 loads in flight before the LDS stores.
@@ -237,10 +266,14 @@ __device__ __forceinline__ bool g2p_finish(const KParams &kp, PtsView pts, int i
         compute_pq(kp, p);
         if (!(p.util & ICEMPM_STATUS_CRUSHED)) check_failure_surface(kp, p);
         crushed = (p.util & ICEMPM_STATUS_CRUSHED) != 0;
+        bool kept = !crushed;
         if (kInlineReturnMap && crushed) {
-            return_map<kDamage>(kp, p);
+            kept = !return_map<kDamage>(kp, p);
             crushed = false;
         }
+        s.have_j = kept;
+        s.Je = p.Je_tr;
+        s.rcJ = p.rc_Jsq;
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) {

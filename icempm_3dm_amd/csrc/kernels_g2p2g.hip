@@ -53,16 +53,24 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
     PointState pf;
     pf.util = ICEMPM_STATUS_DISABLED;
     if (c.valid() && c.c0 + t < c.end) load_g2p_state(in, c.c0 + t, pf);
+    // the segment before was the next brick down in z and left its tiles
+    // moved up by 4 nodes (p2g_flush_lower_slide)
+    bool slid = false;
     while (c.valid()) {
         const KParams ks = scene_view<kEns>(kp, c.brick);
         int org[3], bc[3];
         tile_origin(ks, c.brick, org, bc);
         if (c.first()) {
             // the previous segment's gathers and flush are finished (barrier at its end)
-            VelTileRegs vt;
-            vel_tile_fetch(vt, t, ks, org, grid_a);
-            vel_tile_store(sv, vt, t);
-            p2g_zero_tile(sh, t);
+            if (slid) {
+                vel_tile_slide(sv, t, ks, org, grid_a);
+                if (t == 0) sh.brickmask = 0u;
+            } else {
+                VelTileRegs vt;
+                vel_tile_fetch(vt, t, ks, org, grid_a);
+                vel_tile_store(sv, vt, t);
+                p2g_zero_tile(sh, t);
+            }
             __syncthreads();
         }
         const int i = c.c0 + t;
@@ -75,6 +83,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         s.Jp = pf.Jp;
         const ChunkCursor nx = cursor_next(c, seg);
         bool updated = false, spilled = false;
+        s.have_j = false;
         if (kOOP && (s.util & ICEMPM_STATUS_DISABLED) && i < c.end)
             pts.at(ICEMPM_IDX_UTILITY, i) = bits_to_double(s.util);
         if (!(s.util & ICEMPM_STATUS_DISABLED)) {
@@ -97,13 +106,20 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
             }
         }
         wave_append(spilled, i, spill, spill_count);
-        // P2G record of the new state, straight from registers. (J, J^{-2/3}
-        // and dev(F F^T) of a point whose F the return mapping left as it
-        // was are those of G2P's crush test; keeping them for the record
-        // instead of recomputing them measured 0.2 ms slower per launch on
-        // 16M points: eight more doubles live across the return mapping.)
-        const double Je = updated ? det3(s.Fe) : 1.0;
-        p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rcbrt(Je * Je), s.Bp, s.Jp, org, err);
+        // P2G record of the new state, straight from registers. For a point
+        // whose F the return mapping left as it was, J and J^{-2/3} are those
+        // of G2P's crush test (s.have_j); dev(F F^T) is formed again, since
+        // keeping its six doubles across the return mapping as well measured
+        // 0.2 ms slower per launch on 16M points (register spills).
+        double Je = 1.0, rc = 1.0;
+        if (updated && s.have_j) {
+            Je = s.Je;
+            rc = s.rcJ;
+        } else if (updated) {
+            Je = det3(s.Fe);
+            rc = rcbrt(Je * Je);
+        }
+        p2g_record(sh, t, ks, updated, s.util, s.pos, s.vel, s.Fe, Je, rc, s.Bp, s.Jp, org, err);
         // the record barrier also publishes the scatter's counts of points
         // outside the tile: one barrier fewer per chunk, and none for the
         // out-of-tile list when the counts are all zero
@@ -113,15 +129,18 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         if (nx.valid() && nx.c0 + t < nx.end) load_g2p_state(in, nx.c0 + t, pf);
         p2g_scatter<true>(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
         if (c.last()) {
-            p2g_flush_nodes(sh, t, ks, org, gs_b);
+            // a successor one brick up in z takes the tile over, moved by 4 nodes
+            slid = nx.valid() && nx.brick == c.brick + 1 && bc[2] + 1 < ks.nbz;
+            if (slid) p2g_flush_lower_slide(sh, t, ks, org, gs_b);
+            else p2g_flush_nodes(sh, t, ks, org, gs_b);
             __syncthreads();
             p2g_mark_tile_bricks(sh, t, ks, bc, bl);
             // No barrier here: the next segment's set-up above writes the
             // velocity tile (last read by the gathers, before the record
-            // barrier) and clears the P2G tile (last read by the flush, before
-            // the barrier above), so only wavefront 0's brick marking is still
-            // running when the other wavefronts start it, and the set-up's
-            // barrier joins them.
+            // barrier) and, unless the slide already did, clears the P2G tile
+            // (last read by the flush, before the barrier above), so only
+            // wavefront 0's brick marking is still running when the other
+            // wavefronts start it, and the set-up's barrier joins them.
 #if ICEMPM_SEGMENT_END_BARRIER
             __syncthreads();
 #endif

@@ -380,6 +380,37 @@ __device__ __forceinline__ void p2g_flush_nodes(P2GShared &sh, int t, const KPar
     }
 }
 
+// End of a segment whose successor is the next brick up in z (its tile
+// starts 4 nodes higher): thread t takes node (tx, ty, q) = (t >> 5,
+// (t >> 2) & 7, t & 3) of the tile's lower half, which the successor's tile
+// does not cover, adds it to the grid and marks its brick, then moves its
+// row's element q + 4 down to q and clears q + 4 (each thread touches only
+// elements q and q + 4 of its row, so no barrier is needed in between). The
+// successor starts with the upper half carried over, and each node of a run
+// of z-neighbour bricks goes to HBM once instead of once per brick reaching
+// it. Call after a barrier; the mask is complete after the next barrier.
+static_assert(kP2GThreads == 256 && kTile == 8, "one thread per node of the tile's lower half");
+__device__ __forceinline__ void p2g_flush_lower_slide(P2GShared &sh, int t, const KParams &kp, const int org[3],
+                                                      const GridState &gs) {
+    const int tx = t >> 5, ty = (t >> 2) & 7, q = t & 3;
+    const int lo = tile_index(tx, ty, q);
+    const double m = sh.tile[0][lo];
+    if (m != 0.0) {
+        const long long idx = node_index(kp, org[0] + tx, org[1] + ty, org[2] + q);
+        unsafeAtomicAdd(&gs.grid[idx], m);
+        unsafeAtomicAdd(&gs.grid[kp.grid_pitch + idx], sh.tile[1][lo]);
+        unsafeAtomicAdd(&gs.grid[2 * kp.grid_pitch + idx], sh.tile[2][lo]);
+        unsafeAtomicAdd(&gs.grid[3 * kp.grid_pitch + idx], sh.tile[3][lo]);
+        const int dbx = (tx + 2) >> 2, dby = (ty + 2) >> 2, dbz = (q + 2) >> 2;
+        atomicOr(&sh.brickmask, 1u << ((dbx * 3 + dby) * 3 + dbz));
+    }
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        sh.tile[l][lo] = sh.tile[l][lo + 4];
+        sh.tile[l][lo + 4] = 0.0;
+    }
+}
+
 // Lists the bricks of sh.brickmask (after p2g_flush_nodes and a barrier).
 // Only lanes 0-26, all in wavefront 0, read the mask, so its clear by lane 0
 // (p2g_zero_tile) later in the same wavefront's program needs no barrier in
