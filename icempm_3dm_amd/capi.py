@@ -200,11 +200,11 @@ class Tuning(C.Structure):
     """icempm_Tuning: the launch settings icempm_autotune keeps."""
 
     _fields_ = [("fuse", C.c_int), ("sort_period", C.c_int), ("blocks_per_cu", C.c_int),
-                ("ms_per_substep", C.c_double)]
+                ("points_per_chunk", C.c_int), ("ms_per_substep", C.c_double)]
 
     def as_dict(self) -> dict:
         return {"fuse": bool(self.fuse), "sort_period": self.sort_period, "blocks_per_cu": self.blocks_per_cu,
-                "ms_per_substep": self.ms_per_substep}
+                "points_per_chunk": self.points_per_chunk, "ms_per_substep": self.ms_per_substep}
 
 
 class IcempmError(RuntimeError):
@@ -269,6 +269,7 @@ _PROTOS = {
     "icempm_ensemble_layout": (C.c_int, [_ctxp, C.c_int, _ip, C.POINTER(C.c_longlong)]),
     "icempm_autotune": (C.c_int, [_ctxp, C.c_int, C.POINTER(Tuning)]),
     "icempm_set_blocks_per_cu": (C.c_int, [_ctxp, C.c_int]),
+    "icempm_set_points_per_chunk": (C.c_int, [_ctxp, C.c_int]),
     "icempm_adjoint_forward": (C.c_int, [_ctxp, C.POINTER(AdjointRun), _dp]),
     "icempm_adjoint_backward": (C.c_int, [_ctxp, _dp, _dp, _dp]),
     "icempm_adjoint_positions": (C.c_int, [_ctxp, C.POINTER(C.c_double)]),

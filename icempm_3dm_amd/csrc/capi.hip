@@ -60,7 +60,7 @@ struct TimedLaunch {
 // bytewise, so it is zero-filled before the fields are set.
 struct GraphKeyFields {
     KParams kp;
-    int n, p2g_blocks, g2p_blocks, fuse, sort_first, run;
+    int n, p2g_blocks, g2p_blocks, fuse, chunk, sort_first, run;
     int g, cur0, cur1, cur_pts, clock_slot;
 };
 // A graph's key is the bytes of a GraphKeyFields whose padding is zero. It is
@@ -121,6 +121,7 @@ struct icempm_ctx {
     int g = 0; // the grid the per-phase calls use, and the last one updated
     long long nbricks = 0;
     bool fuse = true;
+    int chunk = 256; // points per chunk of the fused kernel (icempm_set_points_per_chunk)
     double *bins = nullptr;
     long long nbins = 0;
     int *err = nullptr;
@@ -575,7 +576,8 @@ int do_g2p2g(icempm_ctx *ctx, int ga, int gb, int record_pq, bool clocked) {
     SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
     ICEMPM_HIP(launch_g2p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[ga].grid, ctx->gstate(gb),
                             ctx->grids[gb].cur, record_pq, 0, ctx->err, ctx->work, ctx->p2g_blocks,
-                            clocked ? ctx->clock + ctx->clock_slot : nullptr, true, ctx->stream));
+                            clocked ? ctx->clock + ctx->clock_slot : nullptr, true, ctx->stream, nullptr,
+                            ctx->chunk));
     ctx->substeps_since_sort++;
     return tm.finish();
 }
@@ -618,6 +620,7 @@ GraphKey graph_key(const icempm_ctx *ctx, bool sort_first, int run) {
     k.p2g_blocks = ctx->p2g_blocks;
     k.g2p_blocks = ctx->g2p_blocks;
     k.fuse = ctx->fuse ? 1 : 0;
+    k.chunk = ctx->chunk;
     k.sort_first = sort_first ? 1 : 0;
     k.run = run;
     k.g = ctx->g;
@@ -1291,7 +1294,7 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     if (trial_substeps < 1) return fail(ICEMPM_ERR_INVALID_ARG, "trial_substeps must be positive");
     const int n = ctx->n;
     icempm_Tuning keep{ctx->fuse ? 1 : 0, ctx->sort_period, std::max(1, ctx->p2g_blocks / std::max(ctx->n_cu, 1)),
-                       0.0};
+                       ctx->chunk, 0.0};
     if (n == 0) {
         if (best) *best = keep;
         return ICEMPM_OK;
@@ -1342,13 +1345,16 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     (void)hipEventCreate(&ev0);
     (void)hipEventCreate(&ev1);
-    icempm_Tuning top{keep.fuse, keep.sort_period, keep.blocks_per_cu, 1e300};
+    icempm_Tuning top{keep.fuse, keep.sort_period, keep.blocks_per_cu, keep.points_per_chunk, 1e300};
     rc = ICEMPM_OK;
     for (int fuse = 1; fuse >= 0 && rc == ICEMPM_OK; fuse--)
-        for (int period : {10, 20, 50, 100, 200})
+        for (int chunk : {256, 128})
+            for (int period : {10, 20, 50, 100, 200})
             for (int bpc : {8, 16, 32}) {
                 if (rc != ICEMPM_OK) break;
+                if (!fuse && chunk != 256) continue; // the chunk size is the fused kernel's
                 ctx->fuse = fuse != 0;
+                ctx->chunk = chunk;
                 ctx->sort_period = period;
                 ctx->p2g_blocks = bpc * ctx->n_cu;
                 ctx->g2p_blocks = 2 * bpc * ctx->n_cu;
@@ -1370,7 +1376,7 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
                                         hipEventElapsedTime(&ms, ev0, ev1) != hipSuccess))
                     rc = fail(ICEMPM_ERR_HIP, "autotune timing");
                 if (rc == ICEMPM_OK && ms / len < top.ms_per_substep)
-                    top = icempm_Tuning{fuse, period, bpc, (double)ms / len};
+                    top = icempm_Tuning{fuse, period, bpc, chunk, (double)ms / len};
             }
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
@@ -1378,6 +1384,7 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     ctx->graph = graph;
     const icempm_Tuning use = rc == ICEMPM_OK ? top : keep;
     ctx->fuse = use.fuse != 0;
+    ctx->chunk = use.points_per_chunk;
     ctx->sort_period = use.sort_period;
     ctx->p2g_blocks = use.blocks_per_cu * ctx->n_cu;
     ctx->g2p_blocks = 2 * use.blocks_per_cu * ctx->n_cu;
@@ -1392,6 +1399,13 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     if (rc2 != ICEMPM_OK) return rc2;
     ctx->substeps_since_sort = 0;
     if (best) *best = use;
+    return ICEMPM_OK;
+}
+
+int icempm_set_points_per_chunk(icempm_ctx *ctx, int points_per_chunk) {
+    if (!ctx || (points_per_chunk != 128 && points_per_chunk != 256))
+        return fail(ICEMPM_ERR_INVALID_ARG, "points_per_chunk must be 128 or 256");
+    ctx->chunk = points_per_chunk;
     return ICEMPM_OK;
 }
 

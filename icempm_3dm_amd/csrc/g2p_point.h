@@ -36,17 +36,21 @@ __device__ __forceinline__ void load_g2p_state(PtsView pts, int i, PointState &s
     s.Jp = pts.at(ICEMPM_IDX_JP_INV, i);
 }
 
-// Node velocities of a tile: values a thread fetches for nodes t and t+256
-// (fetched ahead, stored to LDS later).
+// Node velocities of a tile: values thread t of an N-thread workgroup
+// fetches for nodes t, t + N, ... (fetched ahead, stored to LDS later).
+template <int N = kP2GThreads>
 struct VelTileRegs {
-    double v[2][3];
+    static_assert(kTileNodes % N == 0, "whole tile nodes per thread");
+    static constexpr int kPer = kTileNodes / N;
+    double v[kPer][3];
 };
 
-__device__ __forceinline__ void vel_tile_fetch(VelTileRegs &r, int t, const KParams &kp, const int org[3],
+template <int N>
+__device__ __forceinline__ void vel_tile_fetch(VelTileRegs<N> &r, int t, const KParams &kp, const int org[3],
                                                const double *grid) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const int k = t + h * kP2GThreads;
+    for (int h = 0; h < VelTileRegs<N>::kPer; h++) {
+        const int k = t + h * N;
         const int x = org[0] + (k >> 6), y = org[1] + ((k >> 3) & 7), z = org[2] + (k & 7);
         r.v[h][0] = r.v[h][1] = r.v[h][2] = 0.0;
         // nodes outside the allocated bricks are never weighted (their points raised an error)
@@ -59,39 +63,52 @@ __device__ __forceinline__ void vel_tile_fetch(VelTileRegs &r, int t, const KPar
     }
 }
 
-__device__ __forceinline__ void vel_tile_store(double (*sv)[kTileAlloc], const VelTileRegs &r, int t) {
+template <int N>
+__device__ __forceinline__ void vel_tile_store(double (*sv)[kTileAlloc], const VelTileRegs<N> &r, int t) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const int k = t + h * kP2GThreads;
+    for (int h = 0; h < VelTileRegs<N>::kPer; h++) {
+        const int k = t + h * N;
         const int tk = tile_index(k >> 6, (k >> 3) & 7, k & 7);
 #pragma unroll
         for (int c = 0; c < 3; c++) sv[c][tk] = r.v[h][c];
     }
 }
-static_assert(kTileNodes == 2 * kP2GThreads, "two tile nodes per thread");
 
 // Start of a segment that follows the next brick down in z: its tile is the
-// previous one moved up 4 nodes. Thread t takes element q = t & 3 of row
-// (tx, ty) = (t >> 5, (t >> 2) & 7): the previous tile's element q + 4 moves
-// down to q, and element q + 4 is read from the grid (node (tx, ty, q + 4) of
-// the new tile). Call after the previous segment's gathers are finished.
+// previous one moved up 4 nodes. For each element n = (tx, ty, q) = (n >> 5,
+// (n >> 2) & 7, n & 3) it handles (n = t, t + N, ...), thread t moves the
+// previous tile's element q + 4 of row (tx, ty) down to q and reads element
+// q + 4 (node (tx, ty, q + 4) of the new tile) from the grid; the reads are
+// all issued before the LDS is touched. Call after the previous segment's
+// gathers are finished.
+template <int N>
 __device__ __forceinline__ void vel_tile_slide(double (*sv)[kTileAlloc], int t, const KParams &kp,
                                                const int org[3], const double *grid) {
-    const int tx = t >> 5, ty = (t >> 2) & 7, q = t & 3;
-    const int x = org[0] + tx, y = org[1] + ty, z = org[2] + 4 + q;
-    double v[3] = {0.0, 0.0, 0.0};
-    // nodes outside the allocated bricks are never weighted (their points raised an error)
-    if ((unsigned)x < (unsigned)(kp.nbx * 4) && (unsigned)y < (unsigned)(kp.nby * 4) &&
-        (unsigned)z < (unsigned)(kp.nbz * 4)) {
-        const long long idx = node_index(kp, x, y, z);
+    constexpr int kPer = kTileNodes / 2 / N;
+    static_assert(kPer * N == kTileNodes / 2, "whole elements per thread");
+    double v[kPer][3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = grid[(1 + c) * kp.grid_pitch + idx];
+    for (int h = 0; h < kPer; h++) {
+        const int n = t + h * N;
+        const int x = org[0] + (n >> 5), y = org[1] + ((n >> 2) & 7), z = org[2] + 4 + (n & 3);
+        v[h][0] = v[h][1] = v[h][2] = 0.0;
+        // nodes outside the allocated bricks are never weighted (their points raised an error)
+        if ((unsigned)x < (unsigned)(kp.nbx * 4) && (unsigned)y < (unsigned)(kp.nby * 4) &&
+            (unsigned)z < (unsigned)(kp.nbz * 4)) {
+            const long long idx = node_index(kp, x, y, z);
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[h][c] = grid[(1 + c) * kp.grid_pitch + idx];
+        }
     }
-    const int lo = tile_index(tx, ty, q);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        sv[c][lo] = sv[c][lo + 4];
-        sv[c][lo + 4] = v[c];
+    for (int h = 0; h < kPer; h++) {
+        const int n = t + h * N;
+        const int lo = tile_index(n >> 5, (n >> 2) & 7, n & 3);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            sv[c][lo] = sv[c][lo + 4];
+            sv[c][lo + 4] = v[h][c];
+        }
     }
 }
 

@@ -120,10 +120,12 @@ hipError_t launch_g2p(const KParams &kp, PtsView pts, int n, SegInfo seg, const 
 // leaves the velocity and C lines unwritten (the next reader is a G2P).
 // With clk (launch_g2p too), record_pq is clk->record. count_zeroed: the
 // spill count (lists[0]) is already zero (cleared by the preceding update).
+// chunk: points per chunk and threads per workgroup, 256 or 128 (128 not for
+// the out-of-place passes of src).
 hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid_a, GridState gs_b,
                         int cur_b, int record_pq, int store_vc, int *err, int *lists, int n_blocks,
                         const StepClock *clk, bool count_zeroed, hipStream_t stream,
-                        const PtsView *src = nullptr);
+                        const PtsView *src = nullptr, int chunk = 256);
 
 // Reverse mode of the step for the calibration adjoint (kernels_adjoint.hip);
 // grids in the step's bricked layout. theta: adjoints of (mu, kappa,

@@ -1,8 +1,8 @@
 // kernels_g2p2g.hip — G2P of substep n fused with P2G of substep n+1.
 //
 // P2G of the next substep depends only on the point state G2P has just
-// produced, so one pass per point does both. Per 256-point chunk, one lane
-// per point:
+// produced, so one pass per point does both. Per chunk of N points (256, or
+// 128 where the autotuner finds it faster), one lane per point:
 //   G  read the 14 doubles G2P needs, gather node velocities from grid A
 //      (LDS tile), update the point (return mapping included) and store it;
 //   P  build the P2G record of the new state from registers, then the
@@ -29,26 +29,28 @@ namespace {
 
 __global__ void k_zero_count(int *c) { *c = 0; }
 
-// LDS (two tiles and the records, about 79 KB) already limits the kernel to
-// two workgroups per CU; the bound keeps the compiler within the matching
-// 256 registers instead of spilling past them into AGPRs.
+// N points per chunk and threads per workgroup. With N = 256, LDS (two
+// tiles and the records, about 78 KB) limits the kernel to two workgroups per
+// CU; with N = 128 (54 KB) three fit. The bound keeps the compiler within the
+// 256 registers of two waves per SIMD instead of spilling past them into
+// AGPRs.
 // kOOP: the point state is read from src and the new state written to pts
 // (the calibration adjoint keeps every substep's state); otherwise both are pts.
 // kDamage: the return mapping with the damage model (kp.dmg_on).
-template <bool kEns, bool kOOP, bool kDamage>
-__global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
+template <bool kEns, bool kOOP, bool kDamage, int N>
+__global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p2g(KParams kp, PtsView pts, SegInfo seg,
                                                        const double *__restrict__ grid_a, GridState gs_b, int cur_b,
                                                        int record_pq, const StepClock *clk, int store_vc, int *err,
                                                        int *spill, int *spill_count, PtsView src) {
     if (clk) record_pq = clk->record;
     const PtsView in = kOOP ? src : pts;
-    __shared__ P2GShared sh;
+    __shared__ P2GSharedT<N> sh;
     __shared__ double sv[3][kTileAlloc];
     const int t = threadIdx.x;
     const BrickList bl = brick_list(gs_b, cur_b);
     const int nseg = *seg.nseg;
     const double c4h = 4. * kp.h_inv;
-    ChunkCursor c = cursor_first(seg, nseg, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
+    ChunkCursorT<N> c = cursor_first<N>(seg, nseg, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
     // what G2P reads of the next chunk's points is loaded during the scatter
     PointState pf;
     pf.util = ICEMPM_STATUS_DISABLED;
@@ -63,10 +65,10 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         if (c.first()) {
             // the previous segment's gathers and flush are finished (barrier at its end)
             if (slid) {
-                vel_tile_slide(sv, t, ks, org, grid_a);
+                vel_tile_slide<N>(sv, t, ks, org, grid_a);
                 if (t == 0) sh.brickmask = 0u;
             } else {
-                VelTileRegs vt;
+                VelTileRegs<N> vt;
                 vel_tile_fetch(vt, t, ks, org, grid_a);
                 vel_tile_store(sv, vt, t);
                 p2g_zero_tile(sh, t);
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
 #pragma unroll
         for (int k = 0; k < 9; k++) s.Fe[k] = pf.Fe[k];
         s.Jp = pf.Jp;
-        const ChunkCursor nx = cursor_next(c, seg);
+        const ChunkCursorT<N> nx = cursor_next(c, seg);
         bool updated = false, spilled = false;
         s.have_j = false;
         if (kOOP && (s.util & ICEMPM_STATUS_DISABLED) && i < c.end)
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
         __syncthreads();
         pf.util = ICEMPM_STATUS_DISABLED;
         if (nx.valid() && nx.c0 + t < nx.end) load_g2p_state(in, nx.c0 + t, pf);
-        p2g_scatter<true>(sh, t, min(kP2GChunk, c.end - c.c0), ks, gs_b, bl);
+        p2g_scatter<true>(sh, t, min(N, c.end - c.c0), ks, gs_b, bl);
         if (c.last()) {
             // a successor one brick up in z takes the tile over, moved by 4 nodes
             slid = nx.valid() && nx.brick == c.brick + 1 && bc[2] + 1 < ks.nbz;
@@ -182,7 +184,11 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p2g_spill(KParams kp, PtsView
 
 hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, const double *grid_a, GridState gs_b,
                         int cur_b, int record_pq, int store_vc, int *err, int *lists, int n_blocks,
-                        const StepClock *clk, bool count_zeroed, hipStream_t stream, const PtsView *src) {
+                        const StepClock *clk, bool count_zeroed, hipStream_t stream, const PtsView *src,
+                        int chunk) {
+    // 128-point chunks are built for the forward step only (the adjoint's
+    // out-of-place passes keep 256)
+    if ((chunk != 128 && chunk != 256) || (src && chunk != 256)) return hipErrorInvalidValue;
     if (n <= 0) return hipSuccess;
     int *spill_count = lists, *spill = lists + 1;
     if (!count_zeroed) k_zero_count<<<1, 1, 0, stream>>>(spill_count);
@@ -190,9 +196,14 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
     if (e != hipSuccess) return e;
     n_blocks = segment_grid(n_blocks);
     const PtsView in = src ? *src : pts;
+#define ICEMPM_G2P2G_N(E, O, D, NT)                                                                              \
+    k_g2p2g<E, O, D, NT><<<n_blocks, NT, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk, store_vc,  \
+                                                      err, spill, spill_count, in)
 #define ICEMPM_G2P2G(E, O, D)                                                                                    \
-    k_g2p2g<E, O, D><<<n_blocks, kP2GThreads, 0, stream>>>(kp, pts, seg, grid_a, gs_b, cur_b, record_pq, clk,       \
-                                                           store_vc, err, spill, spill_count, in)
+    do {                                                                                                         \
+        if (chunk == 128 && !(O)) ICEMPM_G2P2G_N(E, false, D, 128);                                              \
+        else ICEMPM_G2P2G_N(E, O, D, 256);                                                                       \
+    } while (0)
     // the adjoint's out-of-place passes never run with the damage model
     const bool ens = kp.ens_n > 0, dmg = kp.dmg_on != 0;
     if (src && dmg) return hipErrorInvalidValue;
@@ -207,6 +218,7 @@ hipError_t launch_g2p2g(const KParams &kp, PtsView pts, int n, SegInfo seg, cons
         else ICEMPM_G2P2G(false, false, false);
     }
 #undef ICEMPM_G2P2G
+#undef ICEMPM_G2P2G_N
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_g2p2g_spill<<<256, kP2GThreads, 0, stream>>>(kp, pts, grid_a, gs_b, cur_b, record_pq, clk, store_vc, err,
                                                     spill, spill_count, in);

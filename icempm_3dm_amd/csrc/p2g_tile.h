@@ -29,6 +29,10 @@
 
 namespace icempm {
 
+// Points per chunk (= threads per workgroup) of the segment kernels. The
+// P2G machinery below is written for any multiple of 64 up to 256 (template
+// parameter N); the fused kernel also runs with 128-point chunks, whose
+// smaller record block lets three workgroups share a CU.
 constexpr int kP2GThreads = 256;
 constexpr int kP2GChunk = 256;
 constexpr int kTile = 8;
@@ -38,22 +42,25 @@ constexpr int kTileNodes = kTile * kTile * kTile;
 constexpr int kTileSY = kTile;
 constexpr int kTileSX = 67;
 constexpr int kTileAlloc = (kTile - 1) * kTileSX + (kTile - 1) * kTileSY + kTile;
-constexpr int kRec = 22;                  // m | wx[3] wy[3] wz[3] | a[3] | Mh[9]
-constexpr int kRecStride = kP2GChunk + 1; // odd row stride: rows on different banks
-constexpr int kGroups = (kP2GThreads / 64) * 7;
+constexpr int kRec = 22; // m | wx[3] wy[3] wz[3] | a[3] | Mh[9]
+// 9-lane column groups of an N-thread workgroup (7 per wavefront).
+template <int N> constexpr int groups_of() { return (N / 64) * 7; }
 
 __device__ __forceinline__ int tile_index(int tx, int ty, int tz) { return tx * kTileSX + ty * kTileSY + tz; }
 
-struct P2GShared {
+template <int N>
+struct P2GSharedT {
+    static_assert(N % 64 == 0 && N >= 64 && N <= 256, "whole wavefronts, at most 256 points per chunk");
     double tile[4][kTileAlloc];
-    double rec[kRec][kRecStride];
-    int base[kP2GChunk];   // tile-relative base node (packed 3:3:3), -1 none, -2 outside the tile
-    int absb[kP2GChunk];   // absolute base node of points outside the tile (packed 10:10:10)
-    int over[kP2GChunk];   // sub-chunk indices of points outside the tile
-    int wave_count[kP2GThreads / 64];
+    double rec[kRec][N + 1]; // odd row stride: rows on different banks
+    int base[N];             // tile-relative base node (packed 3:3:3), -1 none, -2 outside the tile
+    int absb[N];             // absolute base node of points outside the tile (packed 10:10:10)
+    int over[N];             // sub-chunk indices of points outside the tile
+    int wave_count[N / 64];
     int nover;
     unsigned brickmask;
 };
+using P2GShared = P2GSharedT<kP2GChunk>;
 
 // Appends brick b to the active list once (its flag guards the append).
 struct BrickList {
@@ -85,8 +92,9 @@ __device__ __forceinline__ void tile_origin(const KParams &kp, int brick, int or
 
 // Clears the tile and the brick mask. The mask is cleared by thread 0, a
 // lane of wavefront 0 like every reader of the mask (p2g_mark_tile_bricks).
-__device__ __forceinline__ void p2g_zero_tile(P2GShared &sh, int t) {
-    for (int k = t; k < 4 * kTileAlloc; k += kP2GThreads) (&sh.tile[0][0])[k] = 0.0;
+template <int N>
+__device__ __forceinline__ void p2g_zero_tile(P2GSharedT<N> &sh, int t) {
+    for (int k = t; k < 4 * kTileAlloc; k += N) (&sh.tile[0][0])[k] = 0.0;
     if (t == 0) sh.brickmask = 0u;
 }
 
@@ -94,7 +102,8 @@ __device__ __forceinline__ void p2g_zero_tile(P2GShared &sh, int t) {
 // partition_kernel_p2g up to the scatter, mpm_kernels.cuh:27-81); Je = det Fe
 // and rc = rcbrt(Je^2) are passed in for solid points. Points with a stencil
 // outside the grid raise the error code and get no record.
-__device__ __forceinline__ void p2g_record(P2GShared &sh, int t, const KParams &kp, bool valid, long long util,
+template <int N>
+__device__ __forceinline__ void p2g_record(P2GSharedT<N> &sh, int t, const KParams &kp, bool valid, long long util,
                                            const double pos[3], const double vel[3], const double Fe[9],
                                            double Je, double rc, const double Bp[9], double Jp_inv,
                                            const int org[3], int *err) {
@@ -158,7 +167,8 @@ __device__ __forceinline__ void column_zero(Column &c) {
 
 // Adds record j: W (m, a + Mh o) for o = (ox, oy, k). All 18 record values
 // are read before any is used, so one LDS round trip serves the point.
-__device__ __forceinline__ void column_add(const P2GShared &sh, int j, int ox, int oy, double dox, double doy,
+template <int N>
+__device__ __forceinline__ void column_add(const P2GSharedT<N> &sh, int j, int ox, int oy, double dox, double doy,
                                            Column &c) {
     const double wx = sh.rec[1 + ox][j], wy = sh.rec[4 + oy][j];
     const double m = sh.rec[0][j];
@@ -187,7 +197,8 @@ __device__ __forceinline__ void column_add(const P2GShared &sh, int j, int ox, i
     }
 }
 
-__device__ __forceinline__ void column_to_tile(P2GShared &sh, int packed, int ox, int oy, const Column &c) {
+template <int N>
+__device__ __forceinline__ void column_to_tile(P2GSharedT<N> &sh, int packed, int ox, int oy, const Column &c) {
     const int tn = tile_index((packed & 7) + ox, ((packed >> 3) & 7) + oy, (packed >> 6) & 7);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -199,7 +210,8 @@ __device__ __forceinline__ void column_to_tile(P2GShared &sh, int packed, int ox
 }
 
 // Adds the column's lowest node (z offset 0) to the tile.
-__device__ __forceinline__ void column_node_to_tile(P2GShared &sh, int packed, int ox, int oy, const Column &c) {
+template <int N>
+__device__ __forceinline__ void column_node_to_tile(P2GSharedT<N> &sh, int packed, int ox, int oy, const Column &c) {
     const int tn = tile_index((packed & 7) + ox, ((packed >> 3) & 7) + oy, (packed >> 6) & 7);
     atomicAdd(&sh.tile[0][tn], c.m[0]);
     atomicAdd(&sh.tile[1][tn], c.v0[0]);
@@ -226,7 +238,8 @@ __device__ __forceinline__ void column_shift(Column &c) {
 // Per wavefront, the number of its points outside the tile (after the
 // thread's own p2g_record; the barrier that completes the records also
 // publishes the counts).
-__device__ __forceinline__ void p2g_count_over(P2GShared &sh, int t) {
+template <int N>
+__device__ __forceinline__ void p2g_count_over(P2GSharedT<N> &sh, int t) {
     const unsigned long long om = __ballot(sh.base[t] == -2);
     if ((t & 63) == 0) sh.wave_count[t >> 6] = __popcll(om);
 }
@@ -234,8 +247,8 @@ __device__ __forceinline__ void p2g_count_over(P2GShared &sh, int t) {
 // Scatter of a sub-chunk whose records are complete (call after a barrier;
 // ends with a barrier). nchunk: number of record slots used. kCounted: the
 // caller ran p2g_count_over before that barrier.
-template <bool kCounted = false>
-__device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, const KParams &kp,
+template <bool kCounted = false, int N>
+__device__ __forceinline__ void p2g_scatter(P2GSharedT<N> &sh, int t, int nchunk, const KParams &kp,
                                             const GridState &gs, const BrickList &bl) {
     const int lane = t & 63;
     const int wave = t >> 6;
@@ -248,7 +261,7 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
         // are needed only when some point is outside the tile (the same
         // total everywhere, so the branch is uniform)
         int offo = 0;
-        for (int w = 0; w < kP2GThreads / 64; w++) {
+        for (int w = 0; w < N / 64; w++) {
             const int c = sh.wave_count[w];
             nover_counted += c;
             if (w < wave) offo += c;
@@ -266,7 +279,7 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
         if (over) sh.over[offo + __popcll(om & ((1ull << lane) - 1ull))] = t;
         if (t == 0) {
             int to = 0;
-            for (int w = 0; w < kP2GThreads / 64; w++) to += sh.wave_count[w];
+            for (int w = 0; w < N / 64; w++) to += sh.wave_count[w];
             sh.nover = to;
             count_stat(kp, kStatP2GOutOfTile, (unsigned long long)to);
         }
@@ -280,6 +293,7 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
     const int ox = gq / 3, oy = gq % 3;
     const double dox = ox, doy = oy;
     {
+        constexpr int kGroups = groups_of<N>();
         const int per = (nchunk + kGroups - 1) / kGroups;
         const int j_lo = group_lane ? min(group * per, nchunk) : nchunk;
         const int j_hi = min(j_lo + per, nchunk);
@@ -308,7 +322,7 @@ __device__ __forceinline__ void p2g_scatter(P2GShared &sh, int t, int nchunk, co
         if (cur_pk >= 0) column_to_tile(sh, cur_pk, ox, oy, col);
     }
     const int nover = kCounted ? nover_counted : sh.nover;
-    for (int r = group_lane ? group : nover; r < nover; r += kGroups) {
+    for (int r = group_lane ? group : nover; r < nover; r += groups_of<N>()) {
         const int j = sh.over[r];
         const int ab = sh.absb[j];
         Column col;
@@ -360,9 +374,10 @@ __device__ __forceinline__ void flush_order(int k, int &tx, int &ty, int &tz) {
 // Adds the tile to the grid and collects the touched bricks in
 // sh.brickmask (call after a barrier; the mask is complete, and the tile
 // free, after the next barrier).
-__device__ __forceinline__ void p2g_flush_nodes(P2GShared &sh, int t, const KParams &kp, const int org[3],
+template <int N>
+__device__ __forceinline__ void p2g_flush_nodes(P2GSharedT<N> &sh, int t, const KParams &kp, const int org[3],
                                                 const GridState &gs) {
-    for (int k = t; k < kTileNodes; k += kP2GThreads) {
+    for (int k = t; k < kTileNodes; k += N) {
         int tx, ty, tz;
         flush_order(k, tx, ty, tz);
         const int tk = tile_index(tx, ty, tz);
@@ -381,33 +396,38 @@ __device__ __forceinline__ void p2g_flush_nodes(P2GShared &sh, int t, const KPar
 }
 
 // End of a segment whose successor is the next brick up in z (its tile
-// starts 4 nodes higher): thread t takes node (tx, ty, q) = (t >> 5,
-// (t >> 2) & 7, t & 3) of the tile's lower half, which the successor's tile
-// does not cover, adds it to the grid and marks its brick, then moves its
-// row's element q + 4 down to q and clears q + 4 (each thread touches only
-// elements q and q + 4 of its row, so no barrier is needed in between). The
+// starts 4 nodes higher): for each node n = (tx, ty, q) = (n >> 5,
+// (n >> 2) & 7, n & 3) of the tile's lower half it handles (n = t, t + N,
+// ...), thread t adds the node to the grid and marks its brick, then moves
+// its row's element q + 4 down to q and clears q + 4 (each thread touches
+// only elements q and q + 4 of its rows, so no barrier is needed in
+// between). The
 // successor starts with the upper half carried over, and each node of a run
 // of z-neighbour bricks goes to HBM once instead of once per brick reaching
 // it. Call after a barrier; the mask is complete after the next barrier.
-static_assert(kP2GThreads == 256 && kTile == 8, "one thread per node of the tile's lower half");
-__device__ __forceinline__ void p2g_flush_lower_slide(P2GShared &sh, int t, const KParams &kp, const int org[3],
+static_assert(kTile == 8, "the lower half of the tile is 256 nodes, 4 per row");
+template <int N>
+__device__ __forceinline__ void p2g_flush_lower_slide(P2GSharedT<N> &sh, int t, const KParams &kp, const int org[3],
                                                       const GridState &gs) {
-    const int tx = t >> 5, ty = (t >> 2) & 7, q = t & 3;
-    const int lo = tile_index(tx, ty, q);
-    const double m = sh.tile[0][lo];
-    if (m != 0.0) {
-        const long long idx = node_index(kp, org[0] + tx, org[1] + ty, org[2] + q);
-        unsafeAtomicAdd(&gs.grid[idx], m);
-        unsafeAtomicAdd(&gs.grid[kp.grid_pitch + idx], sh.tile[1][lo]);
-        unsafeAtomicAdd(&gs.grid[2 * kp.grid_pitch + idx], sh.tile[2][lo]);
-        unsafeAtomicAdd(&gs.grid[3 * kp.grid_pitch + idx], sh.tile[3][lo]);
-        const int dbx = (tx + 2) >> 2, dby = (ty + 2) >> 2, dbz = (q + 2) >> 2;
-        atomicOr(&sh.brickmask, 1u << ((dbx * 3 + dby) * 3 + dbz));
-    }
 #pragma unroll
-    for (int l = 0; l < 4; l++) {
-        sh.tile[l][lo] = sh.tile[l][lo + 4];
-        sh.tile[l][lo + 4] = 0.0;
+    for (int n = t; n < kTileNodes / 2; n += N) {
+        const int tx = n >> 5, ty = (n >> 2) & 7, q = n & 3;
+        const int lo = tile_index(tx, ty, q);
+        const double m = sh.tile[0][lo];
+        if (m != 0.0) {
+            const long long idx = node_index(kp, org[0] + tx, org[1] + ty, org[2] + q);
+            unsafeAtomicAdd(&gs.grid[idx], m);
+            unsafeAtomicAdd(&gs.grid[kp.grid_pitch + idx], sh.tile[1][lo]);
+            unsafeAtomicAdd(&gs.grid[2 * kp.grid_pitch + idx], sh.tile[2][lo]);
+            unsafeAtomicAdd(&gs.grid[3 * kp.grid_pitch + idx], sh.tile[3][lo]);
+            const int dbx = (tx + 2) >> 2, dby = (ty + 2) >> 2, dbz = (q + 2) >> 2;
+            atomicOr(&sh.brickmask, 1u << ((dbx * 3 + dby) * 3 + dbz));
+        }
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            sh.tile[l][lo] = sh.tile[l][lo + 4];
+            sh.tile[l][lo + 4] = 0.0;
+        }
     }
 }
 
@@ -420,8 +440,8 @@ __device__ __forceinline__ void p2g_flush_lower_slide(P2GShared &sh, int t, cons
 constexpr int kBrickMaskLanes = 27;
 static_assert(kBrickMaskLanes <= 64, "the brick-mask readers must all lie in wavefront 0");
 static_assert(kBrickMaskLanes <= 32, "the brick mask is one 32-bit word");
-static_assert(kP2GThreads % 64 == 0, "whole wavefronts per workgroup");
-__device__ __forceinline__ void p2g_mark_tile_bricks(const P2GShared &sh, int t, const KParams &kp, const int bc[3],
+template <int N>
+__device__ __forceinline__ void p2g_mark_tile_bricks(const P2GSharedT<N> &sh, int t, const KParams &kp, const int bc[3],
                                                      const BrickList &bl) {
     if (t < kBrickMaskLanes && ((sh.brickmask >> t) & 1u)) {
         const int dbx = t / 9, dby = (t / 3) % 3, dbz = t % 3;
@@ -432,7 +452,8 @@ __device__ __forceinline__ void p2g_mark_tile_bricks(const P2GShared &sh, int t,
 
 // Adds the tile to the grid and marks the touched bricks (call after a
 // barrier; ends with a barrier).
-__device__ __forceinline__ void p2g_flush(P2GShared &sh, int t, const KParams &kp, const int org[3],
+template <int N>
+__device__ __forceinline__ void p2g_flush(P2GSharedT<N> &sh, int t, const KParams &kp, const int org[3],
                                           const int bc[3], const GridState &gs, const BrickList &bl) {
     p2g_flush_nodes(sh, t, kp, org, gs);
     __syncthreads();
@@ -440,17 +461,20 @@ __device__ __forceinline__ void p2g_flush(P2GShared &sh, int t, const KParams &k
     __syncthreads();
 }
 
-// Work items of a workgroup: the sub-chunks of its contiguous range of
-// segments, in order.
-struct ChunkCursor {
+// Work items of a workgroup: the N-point sub-chunks of its contiguous range
+// of segments, in order.
+template <int N>
+struct ChunkCursorT {
     long long s, s_hi;
     int begin, end, c0, brick;
     __device__ __forceinline__ bool valid() const { return s < s_hi; }
     __device__ __forceinline__ bool first() const { return c0 == begin; }
-    __device__ __forceinline__ bool last() const { return c0 + kP2GChunk >= end; }
+    __device__ __forceinline__ bool last() const { return c0 + N >= end; }
 };
+using ChunkCursor = ChunkCursorT<kP2GChunk>;
 
-__device__ __forceinline__ void cursor_load_segment(ChunkCursor &c, const SegInfo &seg) {
+template <int N>
+__device__ __forceinline__ void cursor_load_segment(ChunkCursorT<N> &c, const SegInfo &seg) {
     if (c.s < c.s_hi) {
         c.begin = seg.seg_start[c.s];
         c.end = seg.seg_start[c.s + 1];
@@ -482,10 +506,11 @@ __device__ __forceinline__ long long seg_at_cost(const int32_t *seg_start, int n
 
 // The contiguous range of segments of workgroup lb (equal shares of the total
 // cost, from the split launch_work_split computed for this grid).
-__device__ __forceinline__ ChunkCursor cursor_first(const SegInfo &seg, int nseg, int lb, int nblocks) {
+template <int N = kP2GChunk>
+__device__ __forceinline__ ChunkCursorT<N> cursor_first(const SegInfo &seg, int nseg, int lb, int nblocks) {
     (void)nseg;
     (void)nblocks;
-    ChunkCursor c;
+    ChunkCursorT<N> c;
     c.s = seg.wg_start[lb];
     c.s_hi = seg.wg_start[lb + 1];
     c.begin = c.end = c.c0 = c.brick = 0;
@@ -493,9 +518,10 @@ __device__ __forceinline__ ChunkCursor cursor_first(const SegInfo &seg, int nseg
     return c;
 }
 
-__device__ __forceinline__ ChunkCursor cursor_next(ChunkCursor c, const SegInfo &seg) {
-    if (c.c0 + kP2GChunk < c.end) {
-        c.c0 += kP2GChunk;
+template <int N>
+__device__ __forceinline__ ChunkCursorT<N> cursor_next(ChunkCursorT<N> c, const SegInfo &seg) {
+    if (c.c0 + N < c.end) {
+        c.c0 += N;
     } else {
         c.s++;
         cursor_load_segment(c, seg);

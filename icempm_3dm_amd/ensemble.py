@@ -108,6 +108,9 @@ class BatchedEnsemble:
     def set_fusion(self, on: bool) -> None:
         check(lib().icempm_set_fusion(self.ctx, 1 if on else 0))
 
+    def set_points_per_chunk(self, points_per_chunk: int) -> None:
+        check(lib().icempm_set_points_per_chunk(self.ctx, int(points_per_chunk)))
+
     def set_sort_period(self, period: int) -> None:
         """Regroup the points by grid brick every `period` substeps (0: never)."""
         check(lib().icempm_set_sort_period(self.ctx, int(period)))

@@ -97,9 +97,13 @@ class GPUImplementation6:
     def set_blocks_per_cu(self, blocks_per_cu: int) -> None:
         check(lib().icempm_set_blocks_per_cu(self.ctx, int(blocks_per_cu)))
 
+    def set_points_per_chunk(self, points_per_chunk: int) -> None:
+        check(lib().icempm_set_points_per_chunk(self.ctx, int(points_per_chunk)))
+
     def autotune(self, trial_substeps: int = 10) -> dict:
-        """icempm_autotune: keeps the fastest fusion / sort period / workgroup count
-        for the current points (which it leaves as they are)."""
+        """icempm_autotune: keeps the fastest fusion / sort period / workgroup
+        count / points per chunk for the current points (which it leaves as
+        they are)."""
         t = capi.Tuning()
         check(lib().icempm_autotune(self.ctx, int(trial_substeps), C.byref(t)))
         return t.as_dict()

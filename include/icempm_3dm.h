@@ -351,8 +351,9 @@ int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long l
 
 /* Autotuner (no reference counterpart). Times icempm_step for each
  * combination of G2P+P2G fusion (on/off), brick re-sort period (10, 20, 50,
- * 100, 200 substeps) and workgroups per CU of the brick-segment kernels (8, 16,
- * 32): trial_substeps rounded up to whole periods, each period opening with
+ * 100, 200 substeps), workgroups per CU of the brick-segment kernels (8, 16,
+ * 32) and, with fusion, points per chunk of the fused kernel (256 or 128):
+ * trial_substeps rounded up to whole periods, each period opening with
  * its re-sort, so the sort is paid in every trial; each from the same device
  * state,
  * then restores the points, the indenter accumulators and the error
@@ -362,10 +363,18 @@ int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long l
 /* Workgroups per CU of the brick-segment kernels (P2G, fused G2P+P2G; G2P
  * uses twice as many): 8 after icempm_create. */
 int icempm_set_blocks_per_cu(icempm_ctx *ctx, int blocks_per_cu);
+/* Points per chunk (= threads per workgroup) of the fused G2P+P2G kernel:
+ * 256 (after icempm_create) or 128. 128-point chunks keep more of a
+ * workgroup's lanes busy on brick segments of few points (spread or small
+ * scenes) and fit three workgroups per CU instead of two. No reference
+ * counterpart (the reference's fixed tpb_P2G/tpb_G2P, parameters_sim_3d.cpp:
+ * 39-41, play this part). */
+int icempm_set_points_per_chunk(icempm_ctx *ctx, int points_per_chunk);
 typedef struct icempm_Tuning {
     int fuse;
     int sort_period;
     int blocks_per_cu;
+    int points_per_chunk;
     double ms_per_substep;
 } icempm_Tuning;
 int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best);

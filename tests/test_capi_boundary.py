@@ -40,6 +40,17 @@ def test_params_layout_matches_reference_struct():
     assert off["tpb_P2G"] == 496 and off["animation_threshold_pos"] == 512
 
 
+def test_tuning_layout_matches_header():
+    # icempm_Tuning: four ints (points_per_chunk in the padding before the
+    # double), then ms_per_substep at 16
+    off = {name: getattr(capi.Tuning, name).offset for name, _ in capi.Tuning._fields_}
+    assert C.sizeof(capi.Tuning) == 24
+    assert off == {"fuse": 0, "sort_period": 4, "blocks_per_cu": 8, "points_per_chunk": 12, "ms_per_substep": 16}
+    header = (ROOT / "include" / "icempm_3dm.h").read_text()
+    body = header[header.index("typedef struct icempm_Tuning"):header.index("} icempm_Tuning;")]
+    assert [ln.split()[-1].rstrip(";") for ln in body.splitlines()[1:] if ln.strip()] == list(off)
+
+
 def test_params_reset_and_derived_constants():
     p = SimParams3D().Reset()
     # Reset defaults (parameters_sim_3d.cpp:5-62)

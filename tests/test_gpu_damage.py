@@ -62,8 +62,8 @@ def _compare_points(got_lines, got_util, ref, spread, n, tag):
     assert dd.max() <= 1e-6, (tag, dd.max())
 
 
-@pytest.mark.parametrize("fuse", [True, False])
-def test_damage_step_matches_oracle(require_gpu, oracle_damage, fuse):
+@pytest.mark.parametrize("fuse,chunk", [(True, 256), (False, 256), (True, 128)])
+def test_damage_step_matches_oracle(require_gpu, oracle_damage, fuse, chunk):
     p, hs = _start(MID)
     shape, center, vel = MID.indenter()
     ind = capi.Indenter.make(shape, center, vel)
@@ -72,6 +72,7 @@ def test_damage_step_matches_oracle(require_gpu, oracle_damage, fuse):
     g.set_damage(DAMAGE)
     g.set_sort_period(3)
     g.set_fusion(fuse)
+    g.set_points_per_chunk(chunk)
     nsub = 10
     g.reset_indenter_force_accumulator()
     t = g.step(nsub, 0.0, 0)

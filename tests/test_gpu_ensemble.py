@@ -47,13 +47,14 @@ def _three_scenes():
     return [a, b, c], params, indenters, clouds
 
 
-@pytest.mark.parametrize("fuse", [True, False])
-def test_ensemble_scenes_match_their_own_oracle_runs(require_gpu, fuse):
+@pytest.mark.parametrize("fuse,chunk", [(True, 256), (False, 256), (True, 128)])
+def test_ensemble_scenes_match_their_own_oracle_runs(require_gpu, fuse, chunk):
     sc, params, indenters, clouds = _three_scenes()
     starts = [c.copy() for c in clouds]
     ens = BatchedEnsemble(sc, clouds=clouds, params=params, indenters=indenters)
     assert len({o for o in ens.origin}) == 3
     ens.set_fusion(fuse)
+    ens.set_points_per_chunk(chunk)
     ens.reset_indenter_force_accumulator()
     nsub = 8
     t_dev = ens.step(nsub)
@@ -115,6 +116,7 @@ def test_autotune_keeps_the_state_and_picks_a_setting(require_gpu):
     pts_before = ens.hssoa.lines()[:, :ens.n_points].copy()
     t = ens.autotune(4)
     assert t["sort_period"] in (10, 20, 50, 100, 200) and t["blocks_per_cu"] in (8, 16, 32) and t["ms_per_substep"] > 0
+    assert t["points_per_chunk"] in (128, 256) and (t["fuse"] or t["points_per_chunk"] == 256)
     after = ens.transfer_from_device()
     assert np.array_equal(after.view(np.int64), before.view(np.int64))
     assert np.array_equal(ens.hssoa.lines()[:, :ens.n_points].view(np.int64), pts_before.view(np.int64))

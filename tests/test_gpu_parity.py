@@ -249,9 +249,11 @@ def test_return_map_distorted_f(require_gpu):
 # per field, err(device, oracle) <= max(1e-12, 10 * spread), the spread being
 # the larger of the oracle against itself with another summation order and
 # against itself built without FMA contraction.
-@pytest.mark.parametrize("law,dt,fuse", [("sphere", 2e-6, True), ("cylinder", 2e-6, True), ("sphere", 1e-5, True),
-                                         ("sphere", 2e-6, False), ("cylinder", 1e-5, False)])
-def test_step_matches_oracle(require_gpu, law, dt, fuse):
+@pytest.mark.parametrize("law,dt,fuse,chunk", [("sphere", 2e-6, True, 256), ("cylinder", 2e-6, True, 256),
+                                               ("sphere", 1e-5, True, 256), ("sphere", 2e-6, False, 256),
+                                               ("cylinder", 1e-5, False, 256), ("sphere", 2e-6, True, 128),
+                                               ("cylinder", 1e-5, True, 128)])
+def test_step_matches_oracle(require_gpu, law, dt, fuse, chunk):
     sc = replace(SMALL, dt=dt)
     if law == "sphere":
         p = scenes.make_params(sc)
@@ -267,6 +269,7 @@ def test_step_matches_oracle(require_gpu, law, dt, fuse):
     g = _gpu_with(p, hs, indenter=(shape, center, vel) if law == "sphere" else None)
     g.set_sort_period(3)
     g.set_fusion(fuse)
+    g.set_points_per_chunk(chunk)
     nsub = 8
     g.reset_indenter_force_accumulator()
     t_dev = g.step(nsub, 0.0, 0)

@@ -80,14 +80,15 @@ def _drifting_block(seed: int, scene=None):
     return sc, p, hs, law
 
 
-@pytest.mark.parametrize("fuse", [True, False])
-def test_drift_out_of_tile_paths_match_oracle(require_gpu, fuse):
+@pytest.mark.parametrize("fuse,chunk", [(True, 256), (False, 256), (True, 128)])
+def test_drift_out_of_tile_paths_match_oracle(require_gpu, fuse, chunk):
     sc, p, hs, law = _drifting_block(41)
     ind = capi.Indenter.make(*law)
     start, ref = hs.copy(), hs.copy()
     g = _gpu_with(p, hs, indenter=law)
     g.set_sort_period(0)
     g.set_fusion(fuse)
+    g.set_points_per_chunk(chunk)
     s0 = g.read_stats()
     t_dev = g.step(DRIFT_SUBSTEPS, 0.0, 0)
     stats = g.read_stats()
