@@ -461,21 +461,21 @@ __device__ __forceinline__ void adj_record(P2GShared &sh, int t, bool valid, con
                                            const double alpha[3], const double M[9], const int org[3]) {
     int packed = -1;
     if (valid) {
-        sh.rec[0][t] = 1.0; // the flush writes the nodes whose line 0 is non-zero
+        sh.rec(0, t) = 1.0; // the flush writes the nodes whose line 0 is non-zero
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             double wd[3];
             bspline3(fx[d], wd);
-            sh.rec[1 + 3 * d][t] = wd[0];
-            sh.rec[2 + 3 * d][t] = wd[1];
-            sh.rec[3 + 3 * d][t] = wd[2];
+            sh.rec(1 + 3 * d, t) = wd[0];
+            sh.rec(2 + 3 * d, t) = wd[1];
+            sh.rec(3 + 3 * d, t) = wd[2];
         }
 #pragma unroll
         for (int r = 0; r < 3; r++) {
             // W (alpha + M (o - f)) = W (a + Mh o) with a = alpha - M f, Mh = M
-            sh.rec[10 + r][t] = alpha[r] - (M[3 * r] * fx[0] + M[3 * r + 1] * fx[1] + M[3 * r + 2] * fx[2]);
+            sh.rec(10 + r, t) = alpha[r] - (M[3 * r] * fx[0] + M[3 * r + 1] * fx[1] + M[3 * r + 2] * fx[2]);
 #pragma unroll
-            for (int c = 0; c < 3; c++) sh.rec[13 + 3 * r + c][t] = M[3 * r + c];
+            for (int c = 0; c < 3; c++) sh.rec(13 + 3 * r + c, t) = M[3 * r + c];
         }
         const int rx = base[0] - org[0], ry = base[1] - org[1], rz = base[2] - org[2];
         if ((unsigned)rx <= kTile - 3 && (unsigned)ry <= kTile - 3 && (unsigned)rz <= kTile - 3) {
@@ -519,9 +519,11 @@ __device__ __forceinline__ void adj_gather(const double w[3][3], const double dw
 }
 
 // kPlastic: some point is crushed in the run (else the return-mapping
-// reverse, and its registers, stay out of the kernel).
+// reverse, and its registers, stay out of the kernel). Two workgroups share
+// a CU (LDS); the waves-per-EU bound keeps the kernel within the 256
+// registers that lets both run.
 template <bool kPlastic>
-__global__ __launch_bounds__(kP2GThreads) void k_adj_g2p_bwd(KParams kp, PtsView s0, PtsView s1, PtsView bar,
+__global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_adj_g2p_bwd(KParams kp, PtsView s0, PtsView s1, PtsView bar,
                                                              PtsView part, SegInfo seg, const double *__restrict__ V,
                                                              GridState gb, int cur_b, double *theta) {
     __shared__ P2GShared sh;

@@ -42,7 +42,6 @@ constexpr int kTileNodes = kTile * kTile * kTile;
 constexpr int kTileSY = kTile;
 constexpr int kTileSX = 67;
 constexpr int kTileAlloc = (kTile - 1) * kTileSX + (kTile - 1) * kTileSY + kTile;
-constexpr int kRec = 22; // m | wx[3] wy[3] wz[3] | a[3] | Mh[9]
 // 9-lane column groups of an N-thread workgroup (7 per wavefront).
 template <int N> constexpr int groups_of() { return (N / 64) * 7; }
 
@@ -52,7 +51,22 @@ template <int N>
 struct P2GSharedT {
     static_assert(N % 64 == 0 && N >= 64 && N <= 256, "whole wavefronts, at most 256 points per chunk");
     double tile[4][kTileAlloc];
-    double rec[kRec][N + 1]; // odd row stride: rows on different banks
+    // The 22 record values of each point slot, m | wx[3] wy[3] wz[3] | a[3] |
+    // Mh[9]: the weights wx, wy (each lane of a column group reads its own
+    // one) as rows of doubles, the 16 values every lane of the group reads
+    // as rows of pairs (m wz0) (wz1 wz2) (a0 a1) (a2 Mh0) ... (Mh7 Mh8), so a
+    // lane reads a point with 2 + 8 LDS instructions instead of 18. Odd row
+    // length: consecutive rows start on different banks.
+    double w[6][N + 1];
+    double2 rq[8][N + 1];
+    // value r (0..21, in the order above) of slot t
+    __device__ __forceinline__ double &rec(int r, int t) {
+        if (r >= 1 && r <= 6) return w[r - 1][t];
+        if (r == 0) return rq[0][t].x;
+        if (r == 7) return rq[0][t].y;
+        const int p = (r - 8) / 2 + 1;
+        return ((r - 8) & 1) ? rq[p][t].y : rq[p][t].x;
+    }
     int base[N];             // tile-relative base node (packed 3:3:3), -1 none, -2 outside the tile
     int absb[N];             // absolute base node of points outside the tile (packed 10:10:10)
     int over[N];             // sub-chunk indices of points outside the tile
@@ -127,20 +141,20 @@ __device__ __forceinline__ void p2g_record(P2GSharedT<N> &sh, int t, const KPara
             }
             const double h = kp.h;
             const double fh[3] = {fx[0] * h, fx[1] * h, fx[2] * h};
-            sh.rec[0][t] = m;
+            sh.rec(0, t) = m;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 double wd[3];
                 bspline3(fx[d], wd);
-                sh.rec[1 + 3 * d][t] = wd[0];
-                sh.rec[2 + 3 * d][t] = wd[1];
-                sh.rec[3 + 3 * d][t] = wd[2];
+                sh.rec(1 + 3 * d, t) = wd[0];
+                sh.rec(2 + 3 * d, t) = wd[1];
+                sh.rec(3 + 3 * d, t) = wd[2];
             }
 #pragma unroll
             for (int r = 0; r < 3; r++) {
-                sh.rec[10 + r][t] = vel[r] * m - (sub[3 * r] * fh[0] + sub[3 * r + 1] * fh[1] + sub[3 * r + 2] * fh[2]);
+                sh.rec(10 + r, t) = vel[r] * m - (sub[3 * r] * fh[0] + sub[3 * r + 1] * fh[1] + sub[3 * r + 2] * fh[2]);
 #pragma unroll
-                for (int c = 0; c < 3; c++) sh.rec[13 + 3 * r + c][t] = sub[3 * r + c] * h;
+                for (int c = 0; c < 3; c++) sh.rec(13 + 3 * r + c, t) = sub[3 * r + c] * h;
             }
             const int rx = base[0] - org[0], ry = base[1] - org[1], rz = base[2] - org[2];
             if ((unsigned)rx <= kTile - 3 && (unsigned)ry <= kTile - 3 && (unsigned)rz <= kTile - 3) {
@@ -165,24 +179,43 @@ __device__ __forceinline__ void column_zero(Column &c) {
     for (int k = 0; k < 3; k++) c.m[k] = c.v0[k] = c.v1[k] = c.v2[k] = 0.0;
 }
 
-// Adds record j: W (m, a + Mh o) for o = (ox, oy, k). All 18 record values
-// are read before any is used, so one LDS round trip serves the point.
+// The 18 record values a column lane uses of point j.
+struct ColRec {
+    double wx, wy, m, wz[3], a[3], M[9];
+};
+
 template <int N>
-__device__ __forceinline__ void column_add(const P2GSharedT<N> &sh, int j, int ox, int oy, double dox, double doy,
-                                           Column &c) {
-    const double wx = sh.rec[1 + ox][j], wy = sh.rec[4 + oy][j];
-    const double m = sh.rec[0][j];
-    const double wz[3] = {sh.rec[7][j], sh.rec[8][j], sh.rec[9][j]};
-    double a[3], M[9];
-#pragma unroll
-    for (int r = 0; r < 3; r++) a[r] = sh.rec[10 + r][j];
-#pragma unroll
-    for (int k = 0; k < 9; k++) M[k] = sh.rec[13 + k][j];
-    __builtin_amdgcn_sched_barrier(0);
-    const double wxy = wx * wy;
+__device__ __forceinline__ void column_load(const P2GSharedT<N> &sh, int j, int ox, int oy, ColRec &r) {
+    r.wx = sh.w[ox][j];
+    r.wy = sh.w[3 + oy][j];
+    const double2 q0 = sh.rq[0][j], q1 = sh.rq[1][j], q2 = sh.rq[2][j], q3 = sh.rq[3][j];
+    const double2 q4 = sh.rq[4][j], q5 = sh.rq[5][j], q6 = sh.rq[6][j], q7 = sh.rq[7][j];
+    r.m = q0.x;
+    r.wz[0] = q0.y;
+    r.wz[1] = q1.x;
+    r.wz[2] = q1.y;
+    r.a[0] = q2.x;
+    r.a[1] = q2.y;
+    r.a[2] = q3.x;
+    r.M[0] = q3.y;
+    r.M[1] = q4.x;
+    r.M[2] = q4.y;
+    r.M[3] = q5.x;
+    r.M[4] = q5.y;
+    r.M[5] = q6.x;
+    r.M[6] = q6.y;
+    r.M[7] = q7.x;
+    r.M[8] = q7.y;
+}
+
+// Adds record r: W (m, a + Mh o) for o = (ox, oy, k).
+__device__ __forceinline__ void column_accum(const ColRec &r, double dox, double doy, Column &c) {
+    const double m = r.m;
+    const double *wz = r.wz, *a = r.a, *M = r.M;
+    const double wxy = r.wx * r.wy;
     double base[3];
 #pragma unroll
-    for (int r = 0; r < 3; r++) base[r] = a[r] + M[3 * r] * dox + M[3 * r + 1] * doy;
+    for (int q = 0; q < 3; q++) base[q] = a[q] + M[3 * q] * dox + M[3 * q + 1] * doy;
     const double mz[3] = {M[2], M[5], M[8]}; // Mh(r, 2)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -195,6 +228,17 @@ __device__ __forceinline__ void column_add(const P2GSharedT<N> &sh, int j, int o
         c.v1[k] += W * u1;
         c.v2[k] += W * u2;
     }
+}
+
+// Adds record j (all 18 values requested before any is used, so one LDS
+// round trip serves the point).
+template <int N>
+__device__ __forceinline__ void column_add(const P2GSharedT<N> &sh, int j, int ox, int oy, double dox, double doy,
+                                           Column &c) {
+    ColRec r;
+    column_load(sh, j, ox, oy, r);
+    __builtin_amdgcn_sched_barrier(0);
+    column_accum(r, dox, doy, c);
 }
 
 template <int N>
@@ -301,7 +345,13 @@ __device__ __forceinline__ void p2g_scatter(P2GSharedT<N> &sh, int t, int nchunk
         column_zero(col);
         int cur_pk = -1;
         for (int j = j_lo; j < j_hi; j++) {
+            // the base node and the record are requested together: one LDS
+            // round trip per point (the record of a slot without a point is
+            // read and not used)
             const int pk = sh.base[j];
+            ColRec rj;
+            column_load(sh, j, ox, oy, rj);
+            __builtin_amdgcn_sched_barrier(0);
             if (pk < 0) continue;
             if (pk != cur_pk) {
                 if (cur_pk >= 0) {
@@ -317,7 +367,7 @@ __device__ __forceinline__ void p2g_scatter(P2GSharedT<N> &sh, int t, int nchunk
                 }
                 cur_pk = pk;
             }
-            column_add(sh, j, ox, oy, dox, doy, col);
+            column_accum(rj, dox, doy, col);
         }
         if (cur_pk >= 0) column_to_tile(sh, cur_pk, ox, oy, col);
     }
