@@ -788,6 +788,9 @@ int alloc_common(icempm_ctx *ctx, const icempm_SimParams3D &prms, int n_points_c
     if (nb * kBrickNodes > (1ll << 40) || nb > 0x7FFFFFF)
         return fail(ICEMPM_ERR_INVALID_ARG, "grid too large");
     ctx->nbricks = nb;
+    // the kernels address a point line with a 32-bit byte offset
+    if ((long long)n_points_capacity > kMaxPointPitch - 64)
+        return fail(ICEMPM_ERR_CAPACITY, "n_points_capacity above 2^29 - 64");
     ctx->capacity = n_points_capacity;
     ctx->pitch = ((long long)std::max(n_points_capacity, 1) + 63) / 64 * 64;
     const size_t cap = (size_t)std::max(n_points_capacity, 1);

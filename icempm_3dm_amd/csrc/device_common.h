@@ -122,10 +122,30 @@ __device__ __forceinline__ KParams scene_of_point(const KParams &kp, const doubl
 }
 
 // Particle storage: 28 SoA lines of `pitch` doubles (HostSideSOA line order).
+// A view is a kernel argument (uniform across the workgroup); pitch is at
+// most kMaxPointPitch.
+constexpr long long kMaxPointPitch = 1ll << 29;
 struct PtsView {
     double *lines; // line k at lines + k*pitch
     long long pitch;
-    __device__ __forceinline__ double &at(int line, int i) const { return lines[(long long)line * pitch + i]; }
+    // Element i of line `line`. The line's start is formed in scalar
+    // registers where it is used, and the lane's element is a 32-bit byte
+    // offset from it (the scalar-base + vector-offset form of the global
+    // memory instructions, no vector address arithmetic). The empty asm
+    // statements keep the compiler from computing the starts of all lines
+    // once and holding them across the kernel (28 scalar register pairs,
+    // which in the segment kernels spill into vector registers and come back
+    // with two lane reads per access) and from folding the lane offset into
+    // a 64-bit vector address.
+    __device__ __forceinline__ double &at(int line, int i) const {
+        using GDouble = __attribute__((address_space(1))) double;
+        using GChar = __attribute__((address_space(1))) char;
+        long long p = pitch;
+        asm volatile("" : "+s"(p));
+        GDouble *row = (GDouble *)lines + (long long)line * p;
+        asm volatile("" : "+s"(row));
+        return *(double *)((GChar *)row + (unsigned)i * 8u);
+    }
 };
 
 __device__ __forceinline__ long long node_index(const KParams &kp, int x, int y, int z) {

@@ -116,7 +116,10 @@ int icempm_destroy(icempm_ctx *ctx);
 /* GPU_Implementation6::allocate_arrays (gpu_implementation6.cpp:385-401) and
  * GPU_Partition_3D::allocate (gpu_partition_3d.cu:76-132): device particle
  * storage for n_points_capacity points, the GridXTotal x GridY x GridZ grid
- * (no halo slices: one partition) and IndenterSubdivisions*GridZ*3 bins. */
+ * (no halo slices: one partition) and IndenterSubdivisions*GridZ*3 bins.
+ * ICEMPM_ERR_CAPACITY if n_points_capacity > 2^29 - 64 (536,870,848 points,
+ * 240 GB of point storage: the kernels address a point line with a 32-bit
+ * byte offset). */
 int icempm_allocate(icempm_ctx *ctx, const icempm_SimParams3D *prms, int n_points_capacity);
 
 /* GPU_Partition_3D::update_constants (gpu_partition_3d.cu:135-143): takes a

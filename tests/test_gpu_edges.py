@@ -10,7 +10,8 @@ cloud whose every point is disabled, and points spread one per brick.
                  (bit for bit), and RemoveDisabledAndSort keeps none;
   one per brick  a ragged cloud with one point in each of a set of bricks
                  (every brick segment a single point, every chunk nearly
-                 empty), fused and unfused, against the oracle.
+                 empty), fused and unfused, against the oracle;
+  capacity       a capacity above the point-addressing limit is refused.
 """
 from __future__ import annotations
 
@@ -124,3 +125,19 @@ def test_one_point_per_brick(require_gpu, fuse):
     assert 100 < len(idx) < n // 100
     ragged = _subset(hs, idx)
     _against_oracle(p, ragged, fuse, 6, f"one point per brick fuse={fuse}")
+
+
+def test_capacity_above_the_point_addressing_limit_is_refused(require_gpu):
+    """The kernels address a point line with a 32-bit byte offset: a capacity
+    above 2^29 - 64 points is refused before any point storage is allocated,
+    and the context stays usable."""
+    from icempm_3dm_amd.gpu import GPUImplementation6
+
+    p = scenes.make_params(SC)
+    g = GPUImplementation6(0)
+    g.hssoa = HostSideSOA(64)
+    with pytest.raises(capi.IcempmError) as e:
+        g.allocate_arrays(p, (1 << 29) - 63)
+    assert e.value.code == capi.ERR_CAPACITY
+    g.allocate_arrays(p, 64)
+    g.close()
