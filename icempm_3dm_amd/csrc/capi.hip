@@ -101,6 +101,8 @@ struct icempm_ctx {
     int32_t *scratch = nullptr;
     int32_t *seg_start = nullptr;
     int32_t *seg_brick = nullptr;
+    int32_t *seg_coord = nullptr; // packed brick coordinates of the segments
+    int32_t *seg_slot = nullptr;  // ensemble slots of the segments
     int32_t *nseg = nullptr;
     int32_t *wg_start = nullptr; // segment ranges of the segment kernels' workgroups
     int split_blocks = 0;        // the grid wg_start was computed for (0: stale)
@@ -189,6 +191,7 @@ struct icempm_ctx {
     long long launches[ICEMPM_N_PHASES] = {0};
 
     PtsView view(int which) const { return PtsView{pts[which], pitch}; }
+    SegInfo seginfo() const { return SegInfo{seg_start, seg_brick, nseg, wg_start, seg_coord, seg_slot}; }
     GridState gstate(int gi) const {
         GridState gs;
         gs.grid = grids[gi].grid;
@@ -452,6 +455,8 @@ void free_all(icempm_ctx *ctx) {
     f(ctx->scratch);
     f(ctx->seg_start);
     f(ctx->seg_brick);
+    f(ctx->seg_coord);
+    f(ctx->seg_slot);
     f(ctx->nseg);
     f(ctx->wg_start);
     ctx->wg_start = nullptr;
@@ -462,7 +467,7 @@ void free_all(icempm_ctx *ctx) {
     f(ctx->stats);
     ctx->stats = nullptr;
     f(ctx->grid_xmajor);
-    ctx->scratch = ctx->seg_start = ctx->seg_brick = ctx->nseg = ctx->work = nullptr;
+    ctx->scratch = ctx->seg_start = ctx->seg_brick = ctx->seg_coord = ctx->seg_slot = ctx->nseg = ctx->work = nullptr;
     ctx->bins = nullptr;
     ctx->err = nullptr;
     ctx->grid_xmajor = nullptr;
@@ -499,8 +504,8 @@ int brick_sort(icempm_ctx *ctx) {
                                   ctx->vals[0], n, ctx->stream));
         ctx->cur_pts = nxt;
     }
-    ICEMPM_HIP(launch_brick_segments(ctx->keys[0], n, ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->scratch,
-                                     ctx->stream));
+    ICEMPM_HIP(launch_brick_segments(ctx->kp, ctx->keys[0], n, ctx->seg_start, ctx->seg_brick, ctx->seg_coord,
+                                     ctx->seg_slot, ctx->nseg, ctx->scratch, ctx->stream));
     ctx->split_blocks = 0;
     ICEMPM_TRY(ensure_split(ctx));
     ctx->substeps_since_sort = 0;
@@ -519,7 +524,7 @@ int do_reset_grid(icempm_ctx *ctx, int gi) {
 int do_p2g(icempm_ctx *ctx, int gi) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_P2G);
     ICEMPM_TRY(ensure_split(ctx));
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
+    const SegInfo seg = ctx->seginfo();
     ICEMPM_HIP(launch_p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->gstate(gi), ctx->grids[gi].cur, ctx->err,
                           ctx->p2g_blocks, ctx->stream));
     return tm.finish();
@@ -558,7 +563,7 @@ int do_update(icempm_ctx *ctx, int gi, double cx, double cy, double cz, double t
 int do_g2p(icempm_ctx *ctx, int gi, int record_pq, bool count_zeroed = false, bool clocked = false) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_G2P);
     ICEMPM_TRY(ensure_split(ctx));
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
+    const SegInfo seg = ctx->seginfo();
     ICEMPM_HIP(launch_g2p(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[gi].grid, record_pq, ctx->err,
                           ctx->work, ctx->g2p_blocks, clocked ? ctx->clock + ctx->clock_slot : nullptr, count_zeroed,
                           ctx->stream));
@@ -573,7 +578,7 @@ int do_g2p(icempm_ctx *ctx, int gi, int record_pq, bool count_zeroed = false, bo
 int do_g2p2g(icempm_ctx *ctx, int ga, int gb, int record_pq, bool clocked) {
     PhaseTimer tm(ctx, ICEMPM_PHASE_G2P2G);
     ICEMPM_TRY(ensure_split(ctx));
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
+    const SegInfo seg = ctx->seginfo();
     ICEMPM_HIP(launch_g2p2g(ctx->kp, ctx->view(ctx->cur_pts), ctx->n, seg, ctx->grids[ga].grid, ctx->gstate(gb),
                             ctx->grids[gb].cur, record_pq, 0, ctx->err, ctx->work, ctx->p2g_blocks,
                             clocked ? ctx->clock + ctx->clock_slot : nullptr, true, ctx->stream, nullptr,
@@ -811,6 +816,8 @@ int alloc_common(icempm_ctx *ctx, const icempm_SimParams3D &prms, int n_points_c
     ICEMPM_TRY(dev_alloc(&ctx->scratch, scr, "sort scratch"));
     ICEMPM_TRY(dev_alloc(&ctx->seg_start, cap + 1, "segments"));
     ICEMPM_TRY(dev_alloc(&ctx->seg_brick, cap, "segment bricks"));
+    ICEMPM_TRY(dev_alloc(&ctx->seg_coord, cap, "segment brick coordinates"));
+    ICEMPM_TRY(dev_alloc(&ctx->seg_slot, cap, "segment scene slots"));
     ICEMPM_TRY(dev_alloc(&ctx->nseg, 1, "segment count"));
     // at most 64 workgroups per CU (icempm_set_blocks_per_cu), rounded to the XCDs, plus one
     ICEMPM_TRY(dev_alloc(&ctx->wg_start, (size_t)segment_grid(64 * ctx->n_cu) + 1, "workgroup split"));
@@ -1591,7 +1598,7 @@ int adj_run(icempm_ctx *ctx, int s0, int count, Slot slot, bool keep, bool recor
     const long long gp = ctx->kp.grid_pitch;
     const int S = A.run.sample_every;
     ICEMPM_TRY(ensure_split(ctx));
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
+    const SegInfo seg = ctx->seginfo();
     int ga = 0, gb = 1;
     ICEMPM_TRY(do_reset_grid(ctx, ga));
     ICEMPM_HIP(launch_p2g(ctx->kp, adj_view(slot(0), A.n), A.n, seg, ctx->gstate(ga), ctx->grids[ga].cur, ctx->err,
@@ -1798,7 +1805,7 @@ int icempm_adjoint_backward_ex(icempm_ctx *ctx, const double *dL_dforces, const 
     }
     const PtsView bar = adj_view(A.bar, n), part = adj_view(A.part, n);
     ICEMPM_TRY(ensure_split(ctx));
-    SegInfo seg{ctx->seg_start, ctx->seg_brick, ctx->nseg, ctx->wg_start};
+    const SegInfo seg = ctx->seginfo();
     GridState gb;
     gb.grid = A.vbar;
     gb.brick_flag = A.vbar_flag;

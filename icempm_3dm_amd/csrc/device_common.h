@@ -78,13 +78,14 @@ __device__ __forceinline__ void count_stat(const KParams &kp, int which, unsigne
 constexpr int kRmcThresholdP = 0, kRmcTanPhi = 1, kRmcJeNew = 2, kRmcCbrtJeNew = 3, kRmcDmgRate = 4, kRmcDmgSoft = 5;
 constexpr int kRmcSize = 6;
 
-// Parameter block of the scene that owns grid brick b (ensembles).
-__device__ __forceinline__ KParams scene_of_brick(const KParams &kp, long long b) {
-    const int bz = (int)(b % kp.nbz);
-    const int by = (int)((b / kp.nbz) % kp.nby);
-    const int bx = (int)(b / ((long long)kp.nbz * kp.nby));
-    const int slot = ((bx / kp.ens_slot_b[0]) * kp.ens_nslot[1] + by / kp.ens_slot_b[1]) * kp.ens_nslot[2] +
-                     bz / kp.ens_slot_b[2];
+// Ensemble slot of the brick with coordinates (bx, by, bz).
+__device__ __forceinline__ int ensemble_slot(const KParams &kp, int bx, int by, int bz) {
+    return ((bx / kp.ens_slot_b[0]) * kp.ens_nslot[1] + by / kp.ens_slot_b[1]) * kp.ens_nslot[2] +
+           bz / kp.ens_slot_b[2];
+}
+
+// Parameter block of ensemble slot `slot`.
+__device__ __forceinline__ KParams scene_of_slot(const KParams &kp, int slot) {
     // The table is written by the host only, so it is read through the
     // constant address space: with a workgroup-uniform brick the compiler
     // then uses scalar loads and scalar registers instead of ~40 VGPRs.
@@ -99,10 +100,25 @@ __device__ __forceinline__ KParams scene_of_brick(const KParams &kp, long long b
     return out;
 }
 
+// Parameter block of the scene that owns grid brick b (ensembles).
+__device__ __forceinline__ KParams scene_of_brick(const KParams &kp, long long b) {
+    const int bz = (int)(b % kp.nbz);
+    const int by = (int)((b / kp.nbz) % kp.nby);
+    const int bx = (int)(b / ((long long)kp.nbz * kp.nby));
+    return scene_of_slot(kp, ensemble_slot(kp, bx, by, bz));
+}
+
 // The scene block for work on brick b: kp itself for one scene.
 template <bool kEns>
 __device__ __forceinline__ KParams scene_view(const KParams &kp, long long b) {
     if constexpr (kEns) return scene_of_brick(kp, b);
+    else return kp;
+}
+
+// The same for a brick segment whose ensemble slot is known.
+template <bool kEns>
+__device__ __forceinline__ KParams scene_view_slot(const KParams &kp, int slot) {
+    if constexpr (kEns) return scene_of_slot(kp, slot);
     else return kp;
 }
 

@@ -31,10 +31,13 @@ hipError_t launch_permute(PtsView in, PtsView out, const int32_t *slot_in, int32
 hipError_t launch_unpermute(PtsView in, PtsView out, const int32_t *slot, int n, hipStream_t stream);
 // Brick segments of sorted brick keys: seg_start[s] (s < nseg) first point of
 // the s-th non-empty brick, seg_start[nseg] = number of keyed points,
-// seg_brick[s] its brick id; *nseg_out on the device.
-hipError_t launch_brick_segments(const uint32_t *sorted_keys, int n, int32_t *seg_start,
-                                 int32_t *seg_brick, int32_t *nseg_out, int32_t *scratch,
-                                 hipStream_t stream);
+// seg_brick[s] its brick id, seg_coord[s] its brick coordinates packed
+// bx | by << 8 | bz << 16, seg_slot[s] its ensemble slot (0 for one scene);
+// *nseg_out on the device. The segment kernels read the coordinates and the
+// slot instead of dividing the brick id by the grid's brick counts.
+hipError_t launch_brick_segments(const KParams &kp, const uint32_t *sorted_keys, int n, int32_t *seg_start,
+                                 int32_t *seg_brick, int32_t *seg_coord, int32_t *seg_slot, int32_t *nseg_out,
+                                 int32_t *scratch, hipStream_t stream);
 // *count_out = number of keys != 0xFFFFFFFF in a sorted key array.
 hipError_t launch_count_valid(const uint32_t *sorted_keys, int n, int32_t *count_out, hipStream_t stream);
 hipError_t launch_iota(int32_t *v, int n, hipStream_t stream);
@@ -96,6 +99,8 @@ struct SegInfo {
     const int32_t *seg_brick;
     const int32_t *nseg;      // device int
     const int32_t *wg_start;  // first segment of each workgroup of the segment kernels (+1 entry), launch_work_split
+    const int32_t *seg_coord; // brick coordinates bx | by << 8 | bz << 16 (at most 256 bricks per axis)
+    const int32_t *seg_slot;  // ensembles: the scene slot of the brick (0 for one scene)
 };
 // Workgroups of a segment-kernel launch for a requested count (a multiple of
 // the 8 XCDs).

@@ -535,7 +535,7 @@ __global__ __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(2))
     ChunkCursor c = cursor_first(seg, nseg, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
     while (c.valid()) {
         int org[3], bc[3];
-        tile_origin(kp, c.brick, org, bc);
+        tile_origin_packed(c.coord, org, bc);
         if (c.first()) {
             VelTileRegs vt;
             vel_tile_fetch(vt, t, kp, org, V);

@@ -34,9 +34,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_g2p(KParams kp, PtsView pts, Se
     const double c4h = 4. * kp.h_inv;
     for (long long sg = nseg * w / nw, sg_hi = nseg * (w + 1) / nw; sg < sg_hi; sg++) {
         const int begin = seg.seg_start[sg], end = seg.seg_start[sg + 1];
-        const KParams ks = scene_view<kEns>(kp, seg.seg_brick[sg]);
+        const KParams ks = scene_view_slot<kEns>(kp, seg.seg_slot[sg]);
         int org[3], bc[3];
-        tile_origin(ks, seg.seg_brick[sg], org, bc);
+        tile_origin_packed(seg.seg_coord[sg], org, bc);
         // the previous segment's gathers precede these writes
         wave_lds_sync();
         vel_tile_load_wave(sv, lane, ks, org, grid);

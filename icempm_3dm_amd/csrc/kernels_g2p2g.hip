@@ -59,9 +59,9 @@ __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(2))) void k_g
     // moved up by 4 nodes (p2g_flush_lower_slide)
     bool slid = false;
     while (c.valid()) {
-        const KParams ks = scene_view<kEns>(kp, c.brick);
+        const KParams ks = scene_view_slot<kEns>(kp, c.slot);
         int org[3], bc[3];
-        tile_origin(ks, c.brick, org, bc);
+        tile_origin_packed(c.coord, org, bc);
         if (c.first()) {
             // the previous segment's gathers and flush are finished (barrier at its end)
             if (slid) {

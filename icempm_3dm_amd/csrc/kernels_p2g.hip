@@ -44,9 +44,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(KParams kp, PtsView pts, Se
     P2GInputs in;
     if (c.valid()) load_p2g_inputs(pts, c.c0 + t, c.c0 + t < c.end, in);
     while (c.valid()) {
-        const KParams ks = scene_view<kEns>(kp, c.brick);
+        const KParams ks = scene_view_slot<kEns>(kp, c.slot);
         int org[3], bc[3];
-        tile_origin(ks, c.brick, org, bc);
+        tile_origin_packed(c.coord, org, bc);
         if (c.first()) p2g_zero_tile(sh, t);
         const double Je = (in.util & ICEMPM_STATUS_DISABLED) ? 1.0 : det3(in.Fe);
         p2g_record(sh, t, ks, c.c0 + t < c.end, in.util, in.pos, in.vel, in.Fe, Je, rcbrt(Je * Je), in.Bp, in.Jp,

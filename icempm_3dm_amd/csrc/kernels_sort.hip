@@ -249,13 +249,17 @@ __global__ void k_brick_heads(const uint32_t *keys, int n, int32_t *flags) {
     flags[i] = head;
 }
 
-__global__ void k_brick_write(const uint32_t *keys, int n, const int32_t *flags, const int32_t *pos,
-                              int32_t *seg_start, int32_t *seg_brick) {
+__global__ void k_brick_write(KParams kp, const uint32_t *keys, int n, const int32_t *flags, const int32_t *pos,
+                              int32_t *seg_start, int32_t *seg_brick, int32_t *seg_coord, int32_t *seg_slot) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (flags[i]) {
+        const int b = (int)(keys[i] >> 6);
+        const int bz = b % kp.nbz, by = (b / kp.nbz) % kp.nby, bx = b / (kp.nbz * kp.nby);
         seg_start[pos[i]] = i;
-        seg_brick[pos[i]] = (int32_t)(keys[i] >> 6);
+        seg_brick[pos[i]] = b;
+        seg_coord[pos[i]] = bx | (by << 8) | (bz << 16);
+        seg_slot[pos[i]] = kp.ens_n > 0 ? ensemble_slot(kp, bx, by, bz) : 0;
     }
 }
 
@@ -378,8 +382,9 @@ hipError_t launch_count_valid(const uint32_t *sorted_keys, int n, int32_t *count
     return hipGetLastError();
 }
 
-hipError_t launch_brick_segments(const uint32_t *sorted_keys, int n, int32_t *seg_start, int32_t *seg_brick,
-                                 int32_t *nseg_out, int32_t *scratch, hipStream_t stream) {
+hipError_t launch_brick_segments(const KParams &kp, const uint32_t *sorted_keys, int n, int32_t *seg_start,
+                                 int32_t *seg_brick, int32_t *seg_coord, int32_t *seg_slot, int32_t *nseg_out,
+                                 int32_t *scratch, hipStream_t stream) {
     // scratch layout: flags[n] | pos[n] | n_valid[1] | scan scratch
     int32_t *flags = scratch;
     int32_t *pos = scratch + n;
@@ -391,7 +396,8 @@ hipError_t launch_brick_segments(const uint32_t *sorted_keys, int n, int32_t *se
         k_brick_heads<<<blocks_for(n, 256), 256, 0, stream>>>(sorted_keys, n, flags);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = exclusive_scan(flags, pos, n, scan_scratch, stream)) != hipSuccess) return e;
-        k_brick_write<<<blocks_for(n, 256), 256, 0, stream>>>(sorted_keys, n, flags, pos, seg_start, seg_brick);
+        k_brick_write<<<blocks_for(n, 256), 256, 0, stream>>>(kp, sorted_keys, n, flags, pos, seg_start, seg_brick,
+                                                               seg_coord, seg_slot);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     k_brick_finish<<<1, 1, 0, stream>>>(flags, pos, n, n_valid, seg_start, nseg_out);

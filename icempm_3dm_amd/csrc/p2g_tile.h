@@ -96,6 +96,15 @@ __device__ __forceinline__ void mark_brick(const BrickList &bl, long long b) {
     }
 }
 
+// Origin (node coordinates) of the tile of the brick with packed coordinates
+// `coord` (bx | by << 8 | bz << 16), and its brick coordinates.
+__device__ __forceinline__ void tile_origin_packed(int coord, int org[3], int bc[3]) {
+    bc[0] = coord & 255;
+    bc[1] = (coord >> 8) & 255;
+    bc[2] = coord >> 16;
+    for (int d = 0; d < 3; d++) org[d] = bc[d] * 4 - 2;
+}
+
 // Origin (node coordinates) of the tile of brick `brick`, and its brick coordinates.
 __device__ __forceinline__ void tile_origin(const KParams &kp, int brick, int org[3], int bc[3]) {
     bc[2] = brick % kp.nbz;
@@ -516,7 +525,7 @@ __device__ __forceinline__ void p2g_flush(P2GSharedT<N> &sh, int t, const KParam
 template <int N>
 struct ChunkCursorT {
     long long s, s_hi;
-    int begin, end, c0, brick;
+    int begin, end, c0, brick, coord, slot;
     __device__ __forceinline__ bool valid() const { return s < s_hi; }
     __device__ __forceinline__ bool first() const { return c0 == begin; }
     __device__ __forceinline__ bool last() const { return c0 + N >= end; }
@@ -529,6 +538,8 @@ __device__ __forceinline__ void cursor_load_segment(ChunkCursorT<N> &c, const Se
         c.begin = seg.seg_start[c.s];
         c.end = seg.seg_start[c.s + 1];
         c.brick = seg.seg_brick[c.s];
+        c.coord = seg.seg_coord[c.s];
+        c.slot = seg.seg_slot[c.s];
         c.c0 = c.begin;
     }
 }
@@ -563,7 +574,7 @@ __device__ __forceinline__ ChunkCursorT<N> cursor_first(const SegInfo &seg, int 
     ChunkCursorT<N> c;
     c.s = seg.wg_start[lb];
     c.s_hi = seg.wg_start[lb + 1];
-    c.begin = c.end = c.c0 = c.brick = 0;
+    c.begin = c.end = c.c0 = c.brick = c.coord = c.slot = 0;
     cursor_load_segment(c, seg);
     return c;
 }
