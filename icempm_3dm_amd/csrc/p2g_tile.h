@@ -105,14 +105,6 @@ __device__ __forceinline__ void tile_origin_packed(int coord, int org[3], int bc
     for (int d = 0; d < 3; d++) org[d] = bc[d] * 4 - 2;
 }
 
-// Origin (node coordinates) of the tile of brick `brick`, and its brick coordinates.
-__device__ __forceinline__ void tile_origin(const KParams &kp, int brick, int org[3], int bc[3]) {
-    bc[2] = brick % kp.nbz;
-    bc[1] = (brick / kp.nbz) % kp.nby;
-    bc[0] = brick / (kp.nbz * kp.nby);
-    for (int d = 0; d < 3; d++) org[d] = bc[d] * 4 - 2;
-}
-
 // Clears the tile and the brick mask. The mask is cleared by thread 0, a
 // lane of wavefront 0 like every reader of the mask (p2g_mark_tile_bricks).
 template <int N>
