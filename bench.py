@@ -134,6 +134,20 @@ def cpu_baseline(scene, prms, hs, max_seconds: float, max_steps: int, threads: i
     }
 
 
+def run_unstable(scene, hs, args) -> dict:
+    """The same points at the larger time step args.unstable_dt, where every
+    point crushes and the cloud spreads over more, sparser brick segments."""
+    sub = argparse.Namespace(**vars(args))
+    sub.steps = args.unstable_steps
+    log(f"bench: dt={args.unstable_dt}: {args.warmup} warm-up + {sub.steps} timed steps")
+    el2, _, err2, _, stats2 = run_steps(scene, hs, sub, args.unstable_dt, False)
+    n = hs.size
+    return {"dt": args.unstable_dt, "steps": sub.steps, "value": n * sub.steps / el2,
+            "ms_per_step": el2 * 1e3 / sub.steps, "device_error": err2,
+            "brick_segments_at_end": stats2["segments"], "crushed_fraction": stats2["crushed_fraction"],
+            "launch": stats2["launch"]}
+
+
 def run_steps(scene, hs, args, dt: float, timed: bool):
     """Uploads the points, runs warm-up + timed steps at time step dt;
     returns (elapsed s, phase timings, device error, active nodes)."""
@@ -403,7 +417,7 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: launch settings from icempm_autotune (else --fuse / --sort-period / --blocks-per-cu)")
     ap.add_argument("--blocks-per-cu", type=int, default=8)
-    ap.add_argument("--only", choices=("ensemble", "adjoint", "damage"), default=None,
+    ap.add_argument("--only", choices=("ensemble", "adjoint", "damage", "unstable"), default=None,
                     help="run only this part and print its result (not the bench line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers with hipMemcpy and exit (the copy of known "
@@ -442,6 +456,9 @@ def main() -> int:
     log(f"bench: generating {scene.name} ({scene.n_points} points)")
     hs = scenes.make_points(scene)
     n = hs.size
+    if args.only == "unstable":
+        print(json.dumps({"unstable_dt": run_unstable(scene, hs, args)}), flush=True)
+        return 0
 
     log(f"bench: dt={args.dt}: {args.warmup} warm-up + {args.steps} timed steps")
     elapsed, timings, err, active_nodes, stats = run_steps(scene, hs, args, args.dt, True)
@@ -496,16 +513,7 @@ def main() -> int:
         "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
 
-    unstable = None
-    if args.unstable_dt > 0:
-        sub = argparse.Namespace(**vars(args))
-        sub.steps = args.unstable_steps
-        log(f"bench: dt={args.unstable_dt}: {args.warmup} warm-up + {sub.steps} timed steps")
-        el2, _, err2, _, stats2 = run_steps(scene, hs, sub, args.unstable_dt, False)
-        unstable = {"dt": args.unstable_dt, "steps": sub.steps, "value": n * sub.steps / el2,
-                    "ms_per_step": el2 * 1e3 / sub.steps, "device_error": err2,
-                    "brick_segments_at_end": stats2["segments"], "crushed_fraction": stats2["crushed_fraction"],
-                    "launch": stats2["launch"]}
+    unstable = run_unstable(scene, hs, args) if args.unstable_dt > 0 else None
 
     ppc = None
     if args.ppc_sweep:
