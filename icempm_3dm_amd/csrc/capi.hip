@@ -103,6 +103,8 @@ struct icempm_ctx {
     int32_t *seg_brick = nullptr;
     int32_t *seg_coord = nullptr; // packed brick coordinates of the segments
     int32_t *seg_slot = nullptr;  // ensemble slots of the segments
+    int32_t *brick_coord = nullptr; // packed coordinates of every brick of the stored grid
+    int32_t *brick_slot = nullptr;  // ensemble slot of every brick (ensembles only)
     int32_t *nseg = nullptr;
     int32_t *wg_start = nullptr; // segment ranges of the segment kernels' workgroups
     int split_blocks = 0;        // the grid wg_start was computed for (0: stale)
@@ -199,6 +201,8 @@ struct icempm_ctx {
         gs.brick_list[0] = grids[gi].list[0];
         gs.brick_list[1] = grids[gi].list[1];
         gs.brick_count = grids[gi].count;
+        gs.brick_coord = brick_coord;
+        gs.brick_slot = brick_slot;
         return gs;
     }
 };
@@ -457,6 +461,8 @@ void free_all(icempm_ctx *ctx) {
     f(ctx->seg_brick);
     f(ctx->seg_coord);
     f(ctx->seg_slot);
+    f(ctx->brick_coord);
+    f(ctx->brick_slot);
     f(ctx->nseg);
     f(ctx->wg_start);
     ctx->wg_start = nullptr;
@@ -468,6 +474,7 @@ void free_all(icempm_ctx *ctx) {
     ctx->stats = nullptr;
     f(ctx->grid_xmajor);
     ctx->scratch = ctx->seg_start = ctx->seg_brick = ctx->seg_coord = ctx->seg_slot = ctx->nseg = ctx->work = nullptr;
+    ctx->brick_coord = ctx->brick_slot = nullptr;
     ctx->bins = nullptr;
     ctx->err = nullptr;
     ctx->grid_xmajor = nullptr;
@@ -829,6 +836,9 @@ int alloc_common(icempm_ctx *ctx, const icempm_SimParams3D &prms, int n_points_c
     ICEMPM_TRY(dev_alloc(&ctx->clock, 2, "substep clock"));
     ICEMPM_HIP(launch_clock_set(ctx->clock, 0.0, 0, 1, ctx->stream));
     ctx->clock_slot = 0;
+    ICEMPM_TRY(dev_alloc(&ctx->brick_coord, (size_t)nb, "brick coordinates"));
+    if (ctx->kp.ens_n > 0) ICEMPM_TRY(dev_alloc(&ctx->brick_slot, (size_t)nb, "brick scene slots"));
+    ICEMPM_HIP(launch_brick_info(ctx->kp, ctx->brick_coord, ctx->brick_slot, nb, ctx->stream));
     for (int i = 0; i < 2; i++) ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(i), nb, ctx->stream));
     ctx->g = 0;
     ICEMPM_TRY(zero_bins(ctx));
@@ -1811,6 +1821,8 @@ int icempm_adjoint_backward_ex(icempm_ctx *ctx, const double *dL_dforces, const 
     gb.brick_flag = A.vbar_flag;
     gb.brick_list[0] = gb.brick_list[1] = A.vbar_list;
     gb.brick_count = A.vbar_count;
+    gb.brick_coord = ctx->brick_coord;
+    gb.brick_slot = ctx->brick_slot;
     for (int g = 0; g < 2; g++) {
         ICEMPM_HIP(launch_clear_grid_all(ctx->kp, ctx->gstate(g), ctx->nbricks, ctx->stream));
         ctx->grids[g].cur = 0;

@@ -70,7 +70,17 @@ struct GridState {
     int32_t *brick_flag;   // nbricks flags (1 = in a list)
     int32_t *brick_list[2];
     int32_t *brick_count;  // [2]
+    // per brick of the stored grid: coordinates bx | by << 8 | bz << 16 and
+    // the ensemble slot (nullptr for one scene); written once at allocation
+    // (launch_brick_info), read by the grid update instead of dividing the
+    // brick id by the grid's brick counts and slot sizes
+    const int32_t *brick_coord;
+    const int32_t *brick_slot;
 };
+// brick_coord[b] and, for ensembles, brick_slot[b] for every brick b of the
+// stored grid.
+hipError_t launch_brick_info(const KParams &kp, int32_t *brick_coord, int32_t *brick_slot, long long nbricks,
+                             hipStream_t stream);
 // Zeroes the bricks of list `cur`, clears their flags, zeroes count[cur^1].
 hipError_t launch_reset_grid(const KParams &kp, GridState gs, int cur, hipStream_t stream);
 // Zeroes the whole grid, all flags and both counts (after allocation).

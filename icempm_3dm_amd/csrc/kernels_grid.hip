@@ -54,6 +54,15 @@ __global__ void k_clear_all(KParams kp, GridState gs, long long nbricks) {
     }
 }
 
+__global__ void k_brick_info(KParams kp, int32_t *coord, int32_t *slot, long long nbricks) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nbricks; b += stride) {
+        const int bz = (int)(b % kp.nbz), by = (int)((b / kp.nbz) % kp.nby), bx = (int)(b / ((long long)kp.nbz * kp.nby));
+        coord[b] = bx | (by << 8) | (bz << 16);
+        if (slot) slot[b] = ensemble_slot(kp, bx, by, bz);
+    }
+}
+
 __global__ void k_clock_set(StepClock *clk, double t, long long step, int every) {
     clk->t = t;
     clk->step = step;
@@ -86,12 +95,16 @@ __device__ __forceinline__ void update_bricks(const KParams &kp0, const GridStat
     for (int k0 = blockIdx.x * kWavesPerBlock + wave; k0 < count; k0 += 64 * stride) {
         const int kl = k0 + lane * stride;
         const int mine = kl < count ? list[kl] : 0;
+        // the bricks' coordinates and scene slots, one per lane like the entries
+        const int mine_coord = gs.brick_coord[mine];
+        const int mine_slot = kEns ? gs.brick_slot[mine] : 0;
         // waited for here, once, not at the head of every brick below (where
         // the wait would also cover the previous brick's stores)
-        asm volatile("" ::"v"(mine));
+        asm volatile("" ::"v"(mine), "v"(mine_coord), "v"(mine_slot));
         const int nbatch = min(64, (count - k0 + stride - 1) / stride);
         for (int j = 0; j < nbatch; j++) {
             const int b = __builtin_amdgcn_readlane(mine, j);
+            const int coord = __builtin_amdgcn_readlane(mine_coord, j);
             const long long idx = (long long)b * kBrickNodes + lane;
             const double mass = gs.grid[idx];
             double v[3];
@@ -101,7 +114,7 @@ __device__ __forceinline__ void update_bricks(const KParams &kp0, const GridStat
             // compiler would otherwise move them
             asm volatile("" ::"v"(v[0]), "v"(v[1]), "v"(v[2]));
             if (mass == 0) continue;
-            const KParams kp = scene_view<kEns>(kp0, b);
+            const KParams kp = scene_view_slot<kEns>(kp0, __builtin_amdgcn_readlane(mine_slot, j));
             const double dt = kp.dt;
             const double cellsize = kp.h;
             const bool sphere = kp.ind_shape == ICEMPM_INDENTER_SPHERE;
@@ -113,9 +126,7 @@ __device__ __forceinline__ void update_bricks(const KParams &kp0, const GridStat
                 cz = law_center(kp.ind_c0[2], t, kp.ind_vel[2]);
             }
             double *bins = bins0 + kp.bins_off;
-            const int bz = b % kp.nbz;
-            const int by = (b / kp.nbz) % kp.nby;
-            const int bx = b / (kp.nbz * kp.nby);
+            const int bx = coord & 255, by = (coord >> 8) & 255, bz = coord >> 16;
             const int gs_[3] = {bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
             const double gnpos[3] = {gs_[0] * cellsize, gs_[1] * cellsize, gs_[2] * cellsize};
             // node index within the scene's grid (walls, sticky band, force bins)
@@ -241,6 +252,13 @@ hipError_t launch_update_nodes(const KParams &kp, GridState gs, int cur, double 
                                double *bins, const UpdateExtras &x, hipStream_t stream) {
     if (kp.ens_n > 0) k_update_nodes<true><<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, t, bins, x);
     else k_update_nodes<false><<<1024, kThreads, 0, stream>>>(kp, gs, cur, cx, cy, cz, t, bins, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_info(const KParams &kp, int32_t *brick_coord, int32_t *brick_slot, long long nbricks,
+                             hipStream_t stream) {
+    if (nbricks <= 0) return hipSuccess;
+    k_brick_info<<<1024, 256, 0, stream>>>(kp, brick_coord, brick_slot, nbricks);
     return hipGetLastError();
 }
 
