@@ -1366,6 +1366,29 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
     (void)hipEventCreate(&ev0);
     (void)hipEventCreate(&ev1);
     icempm_Tuning top{keep.fuse, keep.sort_period, keep.blocks_per_cu, keep.points_per_chunk, 1e300};
+    // one trial: ms per substep of whole periods from the saved state, each
+    // period opening with its re-sort (the sort is amortised over the period)
+    auto trial = [&](int fuse, int chunk, int period, int bpc, double *ms_per_substep) -> int {
+        ctx->fuse = fuse != 0;
+        ctx->chunk = chunk;
+        ctx->sort_period = period;
+        ctx->p2g_blocks = bpc * ctx->n_cu;
+        ctx->g2p_blocks = 2 * bpc * ctx->n_cu;
+        ICEMPM_TRY(restore());
+        ICEMPM_TRY(brick_sort(ctx));
+        double t = 0.0;
+        ICEMPM_TRY(icempm_step(ctx, 2, &t, 0));
+        const int len = (trial_substeps + period - 1) / period * period;
+        ctx->substeps_since_sort = period;
+        float ms = 0.f;
+        if (hipEventRecord(ev0, ctx->stream) != hipSuccess) return fail(ICEMPM_ERR_HIP, "hipEventRecord");
+        ICEMPM_TRY(icempm_step(ctx, len, &t, 2));
+        if (hipEventRecord(ev1, ctx->stream) != hipSuccess) return fail(ICEMPM_ERR_HIP, "hipEventRecord");
+        if (hipEventSynchronize(ev1) != hipSuccess || hipEventElapsedTime(&ms, ev0, ev1) != hipSuccess)
+            return fail(ICEMPM_ERR_HIP, "autotune timing");
+        *ms_per_substep = (double)ms / len;
+        return ICEMPM_OK;
+    };
     rc = ICEMPM_OK;
     for (int fuse = 1; fuse >= 0 && rc == ICEMPM_OK; fuse--)
         for (int chunk : {256, 128})
@@ -1373,31 +1396,20 @@ int icempm_autotune(icempm_ctx *ctx, int trial_substeps, icempm_Tuning *best) {
             for (int bpc : {8, 16, 32}) {
                 if (rc != ICEMPM_OK) break;
                 if (!fuse && chunk != 256) continue; // the chunk size is the fused kernel's
-                ctx->fuse = fuse != 0;
-                ctx->chunk = chunk;
-                ctx->sort_period = period;
-                ctx->p2g_blocks = bpc * ctx->n_cu;
-                ctx->g2p_blocks = 2 * bpc * ctx->n_cu;
-                rc = restore();
-                if (rc == ICEMPM_OK) rc = brick_sort(ctx);
-                double t = 0.0;
-                if (rc == ICEMPM_OK) rc = icempm_step(ctx, 2, &t, 0);
-                if (rc != ICEMPM_OK) break;
-                // whole periods, each opening with its re-sort: the timed cost
-                // is the sort amortised over the period plus its substeps
-                const int len = (trial_substeps + period - 1) / period * period;
-                ctx->substeps_since_sort = period;
-                float ms = 0.f;
-                if (hipEventRecord(ev0, ctx->stream) != hipSuccess) rc = fail(ICEMPM_ERR_HIP, "hipEventRecord");
-                if (rc == ICEMPM_OK) rc = icempm_step(ctx, len, &t, 2);
-                if (rc == ICEMPM_OK && hipEventRecord(ev1, ctx->stream) != hipSuccess)
-                    rc = fail(ICEMPM_ERR_HIP, "hipEventRecord");
-                if (rc == ICEMPM_OK && (hipEventSynchronize(ev1) != hipSuccess ||
-                                        hipEventElapsedTime(&ms, ev0, ev1) != hipSuccess))
-                    rc = fail(ICEMPM_ERR_HIP, "autotune timing");
-                if (rc == ICEMPM_OK && ms / len < top.ms_per_substep)
-                    top = icempm_Tuning{fuse, period, bpc, chunk, (double)ms / len};
+                double ms = 0.0;
+                rc = trial(fuse, chunk, period, bpc, &ms);
+                if (rc == ICEMPM_OK && ms < top.ms_per_substep) top = icempm_Tuning{fuse, period, bpc, chunk, ms};
             }
+    // Where the longest period tried wins (points that keep to their tiles),
+    // longer ones are tried with the winning settings, 500 and then 1000
+    // substeps, as long as each one gains.
+    for (int period : {500, 1000}) {
+        if (rc != ICEMPM_OK || top.sort_period != (period == 500 ? 200 : 500)) break;
+        double ms = 0.0;
+        rc = trial(top.fuse, top.points_per_chunk, period, top.blocks_per_cu, &ms);
+        if (rc == ICEMPM_OK && ms < top.ms_per_substep)
+            top = icempm_Tuning{top.fuse, period, top.blocks_per_cu, top.points_per_chunk, ms};
+    }
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     ctx->timing = timing;

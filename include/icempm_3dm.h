@@ -355,10 +355,11 @@ int icempm_ensemble_layout(icempm_ctx *ctx, int scene, int origin_out[3], long l
 /* Autotuner (no reference counterpart). Times icempm_step for each
  * combination of G2P+P2G fusion (on/off), brick re-sort period (10, 20, 50,
  * 100, 200 substeps), workgroups per CU of the brick-segment kernels (8, 16,
- * 32) and, with fusion, points per chunk of the fused kernel (256 or 128):
- * trial_substeps rounded up to whole periods, each period opening with
- * its re-sort, so the sort is paid in every trial; each from the same device
- * state,
+ * 32) and, with fusion, points per chunk of the fused kernel (256 or 128);
+ * where period 200 wins, periods 500 and then 1000 with the winning settings
+ * while each gains. Each trial runs trial_substeps rounded up to whole
+ * periods, each period opening with its re-sort, so the sort is paid in
+ * every trial; each from the same device state,
  * then restores the points, the indenter accumulators and the error
  * indicator and keeps the fastest combination (written to *best if not
  * NULL). The context's simulation time is not advanced. Needs device

@@ -115,7 +115,7 @@ def test_autotune_keeps_the_state_and_picks_a_setting(require_gpu):
     before = ens.transfer_from_device().copy()
     pts_before = ens.hssoa.lines()[:, :ens.n_points].copy()
     t = ens.autotune(4)
-    assert t["sort_period"] in (10, 20, 50, 100, 200) and t["blocks_per_cu"] in (8, 16, 32) and t["ms_per_substep"] > 0
+    assert t["sort_period"] in (10, 20, 50, 100, 200, 500, 1000) and t["blocks_per_cu"] in (8, 16, 32) and t["ms_per_substep"] > 0
     assert t["points_per_chunk"] in (128, 256) and (t["fuse"] or t["points_per_chunk"] == 256)
     after = ens.transfer_from_device()
     assert np.array_equal(after.view(np.int64), before.view(np.int64))
